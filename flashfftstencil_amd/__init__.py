@@ -1,0 +1,268 @@
+"""flashfftstencil_amd -- Python binding of libffstencil, the MI355X tiled FFT-stencil engine.
+
+The product is the C ABI declared in ``include/ffstencil.h`` and implemented by
+``flashfftstencil_amd/libffstencil.so`` (HIP kernels for gfx950 + host plan/run code).  This module is a
+thin ctypes layer over it for the test suite and the benchmark; it computes nothing itself and raises
+when the library is missing.  Device buffers may be passed as raw pointers or as torch CUDA tensors
+(float64, contiguous); torch is only used for device memory and streams.
+
+Mirrors the reference's user-facing pieces (KevinWu2017/FlashFFTStencil):
+  * shape table + weights       -> :func:`named_stencil`  (src/1D/1d_main.cu:44-65, src/2D/2d_main.cu:49-52)
+  * ``CreatePlan(k, K, print)``  -> :class:`Plan`           (src/2D/create_fft_pfa_plan.cu:17)
+  * the launch loop over T       -> :meth:`Plan.run`        (src/2D/2d_main.cu:150-170)
+  * the CSV line                 -> :func:`format_csv`      (src/2D/2d_main.cu:185)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libffstencil.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+FFS_OK = 0
+STATUS_NAMES = {
+    0: "FFS_OK",
+    1: "FFS_ERR_INVALID_ARG",
+    2: "FFS_ERR_UNSUPPORTED",
+    3: "FFS_ERR_OUT_OF_MEMORY",
+    4: "FFS_ERR_HIP",
+    5: "FFS_ERR_NO_DEVICE",
+    6: "FFS_ERR_INTERNAL",
+}
+PRECISIONS = {"fp64": 0, "bf16": 1, "mixed": 1, "f32": 2}
+CONVENTIONS = {"ref": 0, "centred": 1, "centered": 1}
+
+
+class FFSError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_int32),
+        ("radius", ctypes.c_int32),
+        ("sizes", ctypes.c_int64 * 3),
+        ("weights", ctypes.POINTER(ctypes.c_double)),
+        ("steps", ctypes.c_int64),
+        ("precision", ctypes.c_int32),
+        ("convention", ctypes.c_int32),
+        ("tile", ctypes.c_int32),
+        ("fuse", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_int32),
+        ("radius", ctypes.c_int32),
+        ("sizes", ctypes.c_int64 * 3),
+        ("steps", ctypes.c_int64),
+        ("precision", ctypes.c_int32),
+        ("convention", ctypes.c_int32),
+        ("tile", ctypes.c_int32 * 3),
+        ("valid", ctypes.c_int32 * 3),
+        ("fuse", ctypes.c_int32),
+        ("fuse_tail", ctypes.c_int32),
+        ("blocks", ctypes.c_int64),
+        ("tiles", ctypes.c_int64),
+        ("launches", ctypes.c_int64),
+        ("flops_per_block", ctypes.c_double),
+        ("bytes_per_block", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load_library() -> ctypes.CDLL:
+    """Load libffstencil.so (built in-tree by ``__graft_entry__.build()``); raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FFSError(5, f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    # torch-ROCm ships its own libamdhip64.so.7; load it first so that this library binds to the same
+    # HIP runtime instance (same soname) instead of pulling a second copy into the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    sigs = {
+        "ffs_problem_init": [P(Problem)],
+        "ffs_stencil_named": [ctypes.c_char_p, ctypes.c_uint64, P(ctypes.c_int32), P(ctypes.c_int32),
+                              P(ctypes.c_double), ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32],
+        "ffs_fill_uniform": [ctypes.c_uint64, P(ctypes.c_double), ctypes.c_int64],
+        "ffs_plan_create": [P(Problem), P(ctypes.c_void_p)],
+        "ffs_plan_get_info": [ctypes.c_void_p, P(PlanInfo)],
+        "ffs_plan_destroy": [ctypes.c_void_p],
+        "ffs_run": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+        "ffs_run_host": [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double)],
+        "ffs_run_block": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p],
+        "ffs_format_csv": [ctypes.c_char_p, ctypes.c_int32, P(ctypes.c_int64), ctypes.c_int64, ctypes.c_double,
+                           ctypes.c_char_p, ctypes.c_int32],
+    }
+    for name, argtypes in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = ctypes.c_int
+    lib.ffs_last_error.restype = ctypes.c_char_p
+    lib.ffs_last_error.argtypes = []
+    lib.ffs_version.restype = ctypes.c_char_p
+    lib.ffs_version.argtypes = []
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != FFS_OK:
+        raise FFSError(status, load_library().ffs_last_error().decode())
+
+
+def version() -> str:
+    return load_library().ffs_version().decode()
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def named_stencil(shape: str, seed: int = 0x5EED0000) -> Tuple[int, int, np.ndarray, str]:
+    """(dims, radius, weights[(2r+1)^dims], csv tag) of a named stencil."""
+    lib = load_library()
+    dims, radius = ctypes.c_int32(), ctypes.c_int32()
+    w = np.zeros(7 ** 3, dtype=np.float64)
+    tag = ctypes.create_string_buffer(32)
+    _check(lib.ffs_stencil_named(shape.encode(), ctypes.c_uint64(seed), ctypes.byref(dims), ctypes.byref(radius),
+                                 _dptr(w), w.size, tag, 32))
+    k = 2 * radius.value + 1
+    return dims.value, radius.value, w[: k ** dims.value].copy(), tag.value.decode()
+
+
+def fill_uniform(seed: int, n: int) -> np.ndarray:
+    out = np.empty(int(n), dtype=np.float64)
+    _check(load_library().ffs_fill_uniform(ctypes.c_uint64(seed), _dptr(out), ctypes.c_int64(out.size)))
+    return out
+
+
+def format_csv(tag: str, sizes: Sequence[int], steps: int, ms: float) -> str:
+    buf = ctypes.create_string_buffer(256)
+    s = (ctypes.c_int64 * 3)(*([int(x) for x in sizes] + [1] * (3 - len(sizes))))
+    _check(load_library().ffs_format_csv(tag.encode(), len(sizes), s, steps, ms, buf, 256))
+    return buf.value.decode()
+
+
+def _ptr(x) -> int:
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):  # torch tensor
+        if not x.is_cuda or x.dtype.__str__() != "torch.float64" or not x.is_contiguous():
+            raise FFSError(1, "device buffers must be contiguous float64 CUDA tensors")
+        return x.data_ptr()
+    raise TypeError(f"expected a device pointer or torch tensor, got {type(x)!r}")
+
+
+class Plan:
+    """A planned stencil problem (owns device tables, the ping-pong buffer and the cached hipGraph)."""
+
+    def __init__(self, dims: int, sizes: Sequence[int], weights, radius: int, steps: int,
+                 precision: str = "fp64", convention: str = "ref", tile: int = 0, fuse: int = 0,
+                 use_graph: bool = True, device: int = 0):
+        lib = load_library()
+        self._w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).ravel())
+        p = Problem()
+        _check(lib.ffs_problem_init(ctypes.byref(p)))
+        p.dims = dims
+        p.radius = radius
+        sz = [int(s) for s in sizes] + [1] * (3 - len(sizes))
+        for i in range(3):
+            p.sizes[i] = sz[i]
+        p.weights = _dptr(self._w)
+        p.steps = int(steps)
+        p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        p.convention = CONVENTIONS[convention] if isinstance(convention, str) else int(convention)
+        p.tile = tile
+        p.fuse = fuse
+        p.use_graph = 1 if use_graph else 0
+        p.device = device
+        self.problem = p
+        self.sizes = tuple(sz[:dims])
+        self.dims = dims
+        h = ctypes.c_void_p()
+        _check(lib.ffs_plan_create(ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def info(self) -> dict:
+        inf = PlanInfo()
+        _check(load_library().ffs_plan_get_info(self._h, ctypes.byref(inf)))
+        return inf.as_dict()
+
+    def run(self, d_in, d_out, stream=None) -> None:
+        """d_out = S^T(d_in); pointers or float64 CUDA tensors.  stream: int handle, torch stream or None
+        (None = torch's current stream when torch is loaded, else the null stream)."""
+        _check(load_library().ffs_run(self._h, ctypes.c_void_p(_ptr(d_in)), ctypes.c_void_p(_ptr(d_out)),
+                                      ctypes.c_void_p(_stream_handle(stream))))
+
+    def run_block(self, d_in, d_out, tail: bool = False, stream=None) -> None:
+        _check(load_library().ffs_run_block(self._h, ctypes.c_void_p(_ptr(d_in)), ctypes.c_void_p(_ptr(d_out)),
+                                            1 if tail else 0, ctypes.c_void_p(_stream_handle(stream))))
+
+    def run_host(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size != int(np.prod(self.sizes)):
+            raise FFSError(1, f"host array has {x.size} points, plan expects {self.sizes}")
+        out = np.empty_like(x)
+        _check(load_library().ffs_run_host(self._h, _dptr(x), _dptr(out)))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load_library().ffs_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stream_handle(stream) -> Optional[int]:
+    if stream is None:
+        import sys
+        torch = sys.modules.get("torch")
+        if torch is not None and torch.cuda.is_initialized():
+            return torch.cuda.current_stream().cuda_stream
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream

@@ -1,0 +1,549 @@
+// ffs_api.cpp -- host side of libffstencil: validation, named stencils, plan creation, the fused
+// time-block loop and its hipGraph replay.  See include/ffstencil.h for the contract.
+#include "../../include/ffstencil.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ffs_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+ffs_status fail(ffs_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return st;
+}
+
+#define FFS_HIP(call)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess) return fail(FFS_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+double uniform01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * 0x1.0p-53; }
+
+struct NamedStencil {
+  const char* name;
+  const char* alias;
+  const char* tag;  // reference CSV tag (src/1D/1d_main.cu:188-193, src/2D/2d_main.cu:180-183, src/3D/3d_main.cu:180-183)
+  int dims, radius;
+  bool star;  // support = axes through the centre only
+};
+
+const NamedStencil kNamed[] = {
+    {"Heat-1D", nullptr, "1d1r", 1, 1, false},      {"1D5P", nullptr, "1d2r", 1, 2, false},
+    {"1D7P", nullptr, "1d3r", 1, 3, false},         {"Heat-2D", nullptr, "star_2d1r", 2, 1, true},
+    {"Box2D9P", nullptr, "box_2d1r", 2, 1, false},  {"Star2D9P", nullptr, "star_2d2r", 2, 2, true},
+    {"Box2D25P", nullptr, "box_2d2r", 2, 2, false}, {"Heat-3D", nullptr, "star_3d1r", 3, 1, true},
+    {"Box3D27P", "Box3d27P", "box_3d1r", 3, 1, false},
+};
+
+}  // namespace
+
+struct ffs_plan {
+  ffs_problem prob{};
+  std::vector<double> weights;
+  int kind = 0;
+  int dev = 0;
+  int num_cus = 256;
+  int32_t P[3] = {1, 1, 1};      // DFT length per axis (1D: 240 on axis "1")
+  int32_t L[3] = {16, 16, 16};   // per-axis DFT lengths of the operand tables
+  int32_t V[3] = {1, 1, 1};
+  int64_t t = 0, t_tail = 0, nblocks = 0;
+  int64_t tiles[3] = {1, 1, 1};
+  int64_t pairs = 0;
+  int32_t h_main[3] = {0, 0, 0}, h_tail[3] = {0, 0, 0};
+  double* d_ops = nullptr;
+  double* d_H = nullptr;       // [main | tail], each 2*len doubles
+  double* d_taps = nullptr;
+  double* d_scratch = nullptr;  // ping-pong grid
+  int64_t npoints = 0;
+  int64_t hlen = 0;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  const double* g_in = nullptr;
+  double* g_out = nullptr;
+};
+
+namespace {
+
+ffs_status validate(const ffs_problem* p, int64_t* npoints) {
+  if (!p) return fail(FFS_ERR_INVALID_ARG, "problem is NULL");
+  if (p->dims < 1 || p->dims > 3) return fail(FFS_ERR_INVALID_ARG, "dims must be 1, 2 or 3 (got %d)", p->dims);
+  if (p->radius < 0) return fail(FFS_ERR_INVALID_ARG, "radius must be >= 0 (got %d)", p->radius);
+  if (p->steps < 0) return fail(FFS_ERR_INVALID_ARG, "steps must be >= 0 (got %lld)", (long long)p->steps);
+  if (!p->weights) return fail(FFS_ERR_INVALID_ARG, "weights pointer is NULL");
+  if (p->precision < FFS_PREC_FP64 || p->precision > FFS_PREC_MIXED_F32)
+    return fail(FFS_ERR_INVALID_ARG, "unknown precision %d", p->precision);
+  if (p->convention != FFS_CONV_REF && p->convention != FFS_CONV_CENTRED)
+    return fail(FFS_ERR_INVALID_ARG, "unknown convention %d", p->convention);
+  if (p->fuse < 0) return fail(FFS_ERR_INVALID_ARG, "fuse must be >= 0 (got %d)", p->fuse);
+  int64_t n = 1;
+  for (int d = 0; d < p->dims; d++) {
+    if (p->sizes[d] < 1) return fail(FFS_ERR_INVALID_ARG, "size[%d] must be >= 1 (got %lld)", d, (long long)p->sizes[d]);
+    if (n > (int64_t(1) << 40) / p->sizes[d]) return fail(FFS_ERR_INVALID_ARG, "grid too large");
+    n *= p->sizes[d];
+  }
+  const int K = 2 * p->radius + 1;
+  int64_t ntaps = 1;
+  for (int d = 0; d < p->dims; d++) ntaps *= K;
+  for (int64_t i = 0; i < ntaps; i++)
+    if (!std::isfinite(p->weights[i])) return fail(FFS_ERR_INVALID_ARG, "weight %lld is not finite", (long long)i);
+  *npoints = n;
+  return FFS_OK;
+}
+
+// Choose tile family, fused depth and valid widths.
+ffs_status choose_config(ffs_plan* pl) {
+  const ffs_problem& p = pl->prob;
+  const int r = p.radius;
+  if (p.dims == 1) {
+    if (p.tile != 0 && p.tile != 240)
+      return fail(FFS_ERR_UNSUPPORTED, "1D tile must be 240 (15x16 prime-factor window), got %d", p.tile);
+    pl->kind = ffsk::KIND_1DPFA;
+    pl->P[1] = 240;
+    pl->L[0] = 15;
+    pl->L[1] = 16;
+  } else if (p.dims == 2) {
+    if (p.tile != 0 && p.tile != 16) return fail(FFS_ERR_UNSUPPORTED, "2D tile must be 16, got %d", p.tile);
+    pl->kind = ffsk::KIND_2D16;
+    pl->P[0] = pl->P[1] = 16;
+  } else {
+    return fail(FFS_ERR_UNSUPPORTED, "3D is not built yet");
+  }
+  const int Pmain = p.dims == 1 ? 240 : 16;
+  const int vmin = p.dims == 1 ? 16 : 2;
+  // largest admissible depth: V = P - 2 r t >= vmin
+  int64_t tmax = r == 0 ? (int64_t)1 << 40 : (Pmain - vmin) / (2 * r);
+  if (tmax < 1) return fail(FFS_ERR_UNSUPPORTED, "radius %d leaves no valid region in a %d-wide tile", r, Pmain);
+  int64_t tbest;
+  if (p.fuse > 0) {
+    if (p.fuse > tmax)
+      return fail(FFS_ERR_UNSUPPORTED, "fuse %d too deep for radius %d and tile %d (max %lld)", p.fuse, r, Pmain,
+                  (long long)tmax);
+    tbest = p.fuse;
+  } else {
+    // maximise valid point-steps per tile: V^d * t (the per-tile DFT work is fixed)
+    double best = -1;
+    tbest = 1;
+    const int64_t tcap = std::min<int64_t>(tmax, 4096);
+    for (int64_t t = 1; t <= tcap; t++) {
+      const double Vv = Pmain - 2.0 * r * t;
+      const double score = std::pow(Vv, p.dims == 1 ? 1 : p.dims) * t;
+      if (score > best * (1 + 1e-9)) {
+        best = score;
+        tbest = t;
+      }
+    }
+  }
+  const int64_t T = p.steps;
+  if (T == 0) {
+    pl->t = tbest;
+    pl->nblocks = 0;
+    pl->t_tail = 0;
+  } else {
+    if (tbest > T) tbest = T;
+    int64_t nb = (T + tbest - 1) / tbest;
+    if (p.fuse == 0) {  // balance: same number of blocks, shallowest depth that still covers T
+      tbest = (T + nb - 1) / nb;
+      nb = (T + tbest - 1) / tbest;
+    }
+    pl->t = tbest;
+    pl->nblocks = nb;
+    const int64_t tail = T - (nb - 1) * tbest;
+    pl->t_tail = (tail == tbest) ? 0 : tail;
+  }
+  const int hi = p.convention == FFS_CONV_REF ? 2 * r : r;  // largest tap offset of one step
+  for (int d = 0; d < 3; d++) pl->V[d] = 1;
+  if (p.dims == 1) {
+    pl->V[1] = (int32_t)(240 - 2 * r * pl->t);
+    pl->h_main[1] = (int32_t)(hi * pl->t);
+    pl->h_tail[1] = (int32_t)(hi * pl->t_tail);
+    pl->tiles[1] = (p.sizes[0] + pl->V[1] - 1) / pl->V[1];
+    pl->pairs = (pl->tiles[1] + 1) / 2;
+  } else {
+    for (int d = 0; d < p.dims; d++) {
+      pl->V[d] = (int32_t)(16 - 2 * r * pl->t);
+      pl->h_main[d] = (int32_t)(hi * pl->t);
+      pl->h_tail[d] = (int32_t)(hi * pl->t_tail);
+      pl->tiles[d] = (p.sizes[d] + pl->V[d] - 1) / pl->V[d];
+    }
+    if (p.dims == 2) pl->pairs = pl->tiles[0] * ((pl->tiles[1] + 1) / 2);
+  }
+  return FFS_OK;
+}
+
+void release(ffs_plan* pl) {
+  if (!pl) return;
+  if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  if (pl->cap_stream) (void)hipStreamDestroy(pl->cap_stream);
+  if (pl->d_ops) (void)hipFree(pl->d_ops);
+  if (pl->d_H) (void)hipFree(pl->d_H);
+  if (pl->d_taps) (void)hipFree(pl->d_taps);
+  if (pl->d_scratch) (void)hipFree(pl->d_scratch);
+  delete pl;
+}
+
+ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, bool tail) {
+  ffsk::TileArgs a{};
+  a.src = src;
+  a.dst = dst;
+  a.ops = pl->d_ops;
+  a.H = pl->d_H + (tail ? 2 * pl->hlen : 0);
+  a.opsf = nullptr;
+  const ffs_problem& p = pl->prob;
+  if (p.dims == 1) {
+    a.n0 = 1;
+    a.n1 = p.sizes[0];
+    a.n2 = 1;
+  } else {
+    a.n0 = p.sizes[0];
+    a.n1 = p.sizes[1];
+    a.n2 = p.dims == 3 ? p.sizes[2] : 1;
+  }
+  a.tiles0 = pl->tiles[0];
+  a.tiles1 = pl->tiles[1];
+  a.tiles2 = pl->tiles[2];
+  a.pairs = pl->pairs;
+  a.V0 = pl->V[0];
+  a.V1 = pl->V[1];
+  a.V2 = pl->V[2];
+  const int32_t* h = tail ? pl->h_tail : pl->h_main;
+  a.h0 = h[0];
+  a.h1 = h[1];
+  a.h2 = h[2];
+  a.precision = p.precision;
+  return a;
+}
+
+// Enqueue the whole block chain d_in -> ... -> d_out on stream s.
+ffs_status enqueue_chain(ffs_plan* pl, const double* d_in, double* d_out, hipStream_t s) {
+  const int64_t nb = pl->nblocks;
+  const double* src = d_in;
+  for (int64_t b = 0; b < nb; b++) {
+    double* dst = ((nb - 1 - b) % 2 == 0) ? d_out : pl->d_scratch;
+    const bool tail = (b == nb - 1) && pl->t_tail > 0;
+    ffsk::TileArgs a = make_args(pl, src, dst, tail);
+    hipError_t e = ffsk::launch_block(a, pl->kind, s, pl->num_cus);
+    if (e != hipSuccess) return fail(FFS_ERR_HIP, "tile kernel launch failed: %s", hipGetErrorString(e));
+    src = dst;
+  }
+  return FFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ffs_status ffs_problem_init(ffs_problem* p) {
+  if (!p) return fail(FFS_ERR_INVALID_ARG, "problem is NULL");
+  std::memset(p, 0, sizeof *p);
+  p->dims = 2;
+  p->radius = 1;
+  p->sizes[0] = p->sizes[1] = p->sizes[2] = 1;
+  p->steps = 1;
+  p->precision = FFS_PREC_FP64;
+  p->convention = FFS_CONV_REF;
+  p->use_graph = 1;
+  return FFS_OK;
+}
+
+ffs_status ffs_stencil_named(const char* shape, uint64_t seed, int32_t* dims, int32_t* radius, double* weights,
+                             int32_t capacity, char* tag, int32_t tag_capacity) {
+  if (!shape) return fail(FFS_ERR_INVALID_ARG, "shape is NULL");
+  const NamedStencil* ns = nullptr;
+  for (const NamedStencil& n : kNamed)
+    if (std::strcmp(shape, n.name) == 0 || (n.alias && std::strcmp(shape, n.alias) == 0)) ns = &n;
+  if (!ns) return fail(FFS_ERR_INVALID_ARG, "unknown stencil shape '%s'", shape);
+  const int K = 2 * ns->radius + 1;
+  int ntaps = 1;
+  for (int d = 0; d < ns->dims; d++) ntaps *= K;
+  if (dims) *dims = ns->dims;
+  if (radius) *radius = ns->radius;
+  if (tag) {
+    if (tag_capacity < (int32_t)std::strlen(ns->tag) + 1) return fail(FFS_ERR_INVALID_ARG, "tag buffer too small");
+    std::strcpy(tag, ns->tag);
+  }
+  if (!weights) return FFS_OK;
+  if (capacity < ntaps) return fail(FFS_ERR_INVALID_ARG, "weights capacity %d < %d taps", capacity, ntaps);
+  if (std::strcmp(ns->name, "Heat-1D") == 0) {
+    weights[0] = 0.25;
+    weights[1] = 0.5;
+    weights[2] = 0.25;
+    return FFS_OK;
+  }
+  uint64_t s = seed;
+  double sum = 0.0;
+  for (int i = 0; i < ntaps; i++) {
+    int rem = i, off = 0;
+    for (int d = 0; d < ns->dims; d++) {
+      const int a = rem % K - ns->radius;
+      rem /= K;
+      if (a != 0) off++;
+    }
+    const bool on = !ns->star || off <= 1;
+    weights[i] = on ? uniform01(s) : 0.0;
+    sum += weights[i];
+  }
+  if (!(sum > 0.0)) return fail(FFS_ERR_INTERNAL, "degenerate weights");
+  for (int i = 0; i < ntaps; i++) weights[i] /= sum;
+  return FFS_OK;
+}
+
+ffs_status ffs_fill_uniform(uint64_t seed, double* host_out, int64_t n) {
+  if (!host_out && n > 0) return fail(FFS_ERR_INVALID_ARG, "output is NULL");
+  if (n < 0) return fail(FFS_ERR_INVALID_ARG, "negative count");
+  uint64_t s = seed;
+  for (int64_t i = 0; i < n; i++) host_out[i] = uniform01(s);
+  return FFS_OK;
+}
+
+ffs_status ffs_format_csv(const char* tag, int32_t dims, const int64_t* sizes, int64_t steps, double ms, char* buf,
+                          int32_t capacity) {
+  if (!tag || !sizes || !buf || capacity <= 0) return fail(FFS_ERR_INVALID_ARG, "NULL argument");
+  if (dims < 1 || dims > 3) return fail(FFS_ERR_INVALID_ARG, "dims must be 1..3");
+  long long d[3] = {1, 1, 1};
+  double pts = 1.0;
+  for (int i = 0; i < dims; i++) {
+    d[3 - dims + i] = (long long)sizes[i];
+    pts *= (double)sizes[i];
+  }
+  const double gst = ms > 0 ? pts * (double)steps / ms / 1e6 : 0.0;
+  const int n = std::snprintf(buf, (size_t)capacity, "FlashFFTStencil, %s, %lld, %lld, %lld, %lld, %g, %g", tag, d[0],
+                              d[1], d[2], (long long)steps, ms, gst);
+  if (n < 0 || n >= capacity) return fail(FFS_ERR_INVALID_ARG, "buffer too small");
+  return FFS_OK;
+}
+
+ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
+  if (!out) return fail(FFS_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  int64_t npoints = 0;
+  ffs_status st = validate(p, &npoints);
+  if (st != FFS_OK) return st;
+  ffs_plan* pl = new (std::nothrow) ffs_plan();
+  if (!pl) return fail(FFS_ERR_OUT_OF_MEMORY, "host allocation failed");
+  pl->prob = *p;
+  const int K = 2 * p->radius + 1;
+  int64_t ntaps = 1;
+  for (int d = 0; d < p->dims; d++) ntaps *= K;
+  pl->weights.assign(p->weights, p->weights + ntaps);
+  pl->prob.weights = pl->weights.data();
+  pl->npoints = npoints;
+  st = choose_config(pl);
+  if (st != FFS_OK) {
+    release(pl);
+    return st;
+  }
+  if (pl->prob.precision != FFS_PREC_FP64) {
+    release(pl);
+    return fail(FFS_ERR_UNSUPPORTED, "mixed precision forms are not built yet");
+  }
+  // ---- device work starts here ----
+  int ndev = 0;
+  const hipError_t ce = hipGetDeviceCount(&ndev);
+  if (ce != hipSuccess || ndev <= 0) {
+    release(pl);
+    return fail(FFS_ERR_NO_DEVICE, "no HIP device visible (%s, count %d)", hipGetErrorString(ce), ndev);
+  }
+  if (p->device < 0 || p->device >= ndev) {
+    release(pl);
+    return fail(FFS_ERR_INVALID_ARG, "device %d out of range (%d devices)", p->device, ndev);
+  }
+  pl->dev = p->device;
+  hipDeviceProp_t prop;
+  if (hipSetDevice(pl->dev) != hipSuccess || hipGetDeviceProperties(&prop, pl->dev) != hipSuccess) {
+    release(pl);
+    return fail(FFS_ERR_HIP, "cannot query device %d", pl->dev);
+  }
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    release(pl);
+    return fail(FFS_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950 only", pl->dev, prop.gcnArchName);
+  }
+  pl->num_cus = prop.multiProcessorCount;
+  pl->hlen = ffsk::spectrum_len(pl->kind);
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) {
+    if (x != hipSuccess && e == hipSuccess) e = x;
+  };
+  chk(hipMalloc(&pl->d_ops, sizeof(double) * ffsk::kOpsLen));
+  chk(hipMalloc(&pl->d_H, sizeof(double) * 4 * pl->hlen));
+  chk(hipMalloc(&pl->d_taps, sizeof(double) * ntaps));
+  if (pl->nblocks >= 2) chk(hipMalloc(&pl->d_scratch, sizeof(double) * npoints));
+  if (e != hipSuccess) {
+    release(pl);
+    return fail(e == hipErrorOutOfMemory ? FFS_ERR_OUT_OF_MEMORY : FFS_ERR_HIP, "device allocation failed: %s",
+                hipGetErrorString(e));
+  }
+  chk(hipMemcpy(pl->d_taps, pl->weights.data(), sizeof(double) * ntaps, hipMemcpyHostToDevice));
+  chk(ffsk::gen_operands(pl->d_ops, nullptr, pl->L, nullptr));
+  chk(ffsk::gen_spectrum(pl->d_H, pl->kind, pl->L, pl->d_taps, K, p->dims, p->convention, p->radius, pl->t, nullptr));
+  chk(ffsk::gen_spectrum(pl->d_H + 2 * pl->hlen, pl->kind, pl->L, pl->d_taps, K, p->dims, p->convention, p->radius,
+                        pl->t_tail > 0 ? pl->t_tail : pl->t, nullptr));
+  chk(hipDeviceSynchronize());
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&pl->cap_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    release(pl);
+    return fail(FFS_ERR_HIP, "plan set-up failed: %s", hipGetErrorString(e));
+  }
+  *out = pl;
+  return FFS_OK;
+}
+
+ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
+  if (!pl || !info) return fail(FFS_ERR_INVALID_ARG, "NULL argument");
+  std::memset(info, 0, sizeof *info);
+  const ffs_problem& p = pl->prob;
+  info->dims = p.dims;
+  info->radius = p.radius;
+  for (int d = 0; d < 3; d++) info->sizes[d] = d < p.dims ? p.sizes[d] : 1;
+  info->steps = p.steps;
+  info->precision = p.precision;
+  info->convention = p.convention;
+  if (p.dims == 1) {
+    info->tile[0] = 240;
+    info->valid[0] = pl->V[1];
+    info->tile[1] = info->tile[2] = 1;
+    info->valid[1] = info->valid[2] = 1;
+    info->tiles = pl->tiles[1];
+  } else {
+    info->tiles = 1;
+    for (int d = 0; d < 3; d++) {
+      info->tile[d] = d < p.dims ? 16 : 1;
+      info->valid[d] = d < p.dims ? pl->V[d] : 1;
+      if (d < p.dims) info->tiles *= pl->tiles[d];
+    }
+  }
+  info->fuse = (int32_t)pl->t;
+  info->fuse_tail = (int32_t)pl->t_tail;
+  info->blocks = pl->nblocks;
+  info->launches = pl->nblocks;
+  // MFMA flops per tile pair: stages x 3 real GEMMs (Gauss form) x 2*16^3 per 16x16x16 GEMM x 16^(d-2) slices
+  const double gemm = 2.0 * 16 * 16 * 16;
+  const int dd = p.dims == 1 ? 2 : p.dims;
+  const double per_pair = 2.0 * dd * 3.0 * gemm * std::pow(16.0, dd - 2);
+  info->flops_per_block = per_pair * (double)pl->pairs;
+  info->bytes_per_block = 16.0 * (double)pl->npoints;
+  return FFS_OK;
+}
+
+ffs_status ffs_plan_destroy(ffs_plan* pl) {
+  if (!pl) return FFS_OK;
+  (void)hipSetDevice(pl->dev);
+  (void)hipDeviceSynchronize();
+  release(pl);
+  return FFS_OK;
+}
+
+ffs_status ffs_run(ffs_plan* pl, const double* d_in, double* d_out, void* stream) {
+  if (!pl || !d_in || !d_out) return fail(FFS_ERR_INVALID_ARG, "NULL argument");
+  if (d_in == d_out) return fail(FFS_ERR_INVALID_ARG, "d_in must not alias d_out");
+  hipStream_t s = (hipStream_t)stream;
+  FFS_HIP(hipSetDevice(pl->dev));
+  if (pl->nblocks == 0) {
+    FFS_HIP(hipMemcpyAsync(d_out, d_in, sizeof(double) * pl->npoints, hipMemcpyDeviceToDevice, s));
+    return FFS_OK;
+  }
+  if (!pl->prob.use_graph) return enqueue_chain(pl, d_in, d_out, s);
+  if (!pl->gexec || pl->g_in != d_in || pl->g_out != d_out) {
+    if (pl->gexec) {
+      FFS_HIP(hipGraphExecDestroy(pl->gexec));
+      pl->gexec = nullptr;
+    }
+    if (pl->graph) {
+      FFS_HIP(hipGraphDestroy(pl->graph));
+      pl->graph = nullptr;
+    }
+    FFS_HIP(hipStreamBeginCapture(pl->cap_stream, hipStreamCaptureModeThreadLocal));
+    ffs_status st = enqueue_chain(pl, d_in, d_out, pl->cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(pl->cap_stream, &g);
+    if (st != FFS_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (ee != hipSuccess) return fail(FFS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ee));
+    pl->graph = g;
+    FFS_HIP(hipGraphInstantiate(&pl->gexec, pl->graph, nullptr, nullptr, 0));
+    pl->g_in = d_in;
+    pl->g_out = d_out;
+  }
+  FFS_HIP(hipGraphLaunch(pl->gexec, s));
+  return FFS_OK;
+}
+
+ffs_status ffs_run_block(ffs_plan* pl, const double* d_in, double* d_out, int32_t tail, void* stream) {
+  if (!pl || !d_in || !d_out) return fail(FFS_ERR_INVALID_ARG, "NULL argument");
+  if (d_in == d_out) return fail(FFS_ERR_INVALID_ARG, "d_in must not alias d_out");
+  if (pl->nblocks == 0) return fail(FFS_ERR_INVALID_ARG, "plan has no blocks (T = 0)");
+  FFS_HIP(hipSetDevice(pl->dev));
+  const bool use_tail = tail != 0 && pl->t_tail > 0;
+  ffsk::TileArgs a = make_args(pl, d_in, d_out, use_tail);
+  hipError_t e = ffsk::launch_block(a, pl->kind, (hipStream_t)stream, pl->num_cus);
+  if (e != hipSuccess) return fail(FFS_ERR_HIP, "tile kernel launch failed: %s", hipGetErrorString(e));
+  return FFS_OK;
+}
+
+ffs_status ffs_run_host(ffs_plan* pl, const double* h_in, double* h_out) {
+  if (!pl || !h_in || !h_out) return fail(FFS_ERR_INVALID_ARG, "NULL argument");
+  FFS_HIP(hipSetDevice(pl->dev));
+  double *din = nullptr, *dout = nullptr;
+  const size_t bytes = sizeof(double) * pl->npoints;
+  hipError_t e = hipMalloc(&din, bytes);
+  if (e == hipSuccess) e = hipMalloc(&dout, bytes);
+  if (e != hipSuccess) {
+    if (din) (void)hipFree(din);
+    return fail(FFS_ERR_OUT_OF_MEMORY, "device allocation failed: %s", hipGetErrorString(e));
+  }
+  ffs_status st = FFS_OK;
+  e = hipMemcpy(din, h_in, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    st = ffs_run(pl, din, dout, nullptr);
+    if (st == FFS_OK) {
+      e = hipStreamSynchronize(nullptr);
+      if (e == hipSuccess) e = hipMemcpy(h_out, dout, bytes, hipMemcpyDeviceToHost);
+    }
+  }
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  // the cached graph referenced the temporaries: drop it
+  if (pl->gexec && pl->g_in == din) {
+    (void)hipGraphExecDestroy(pl->gexec);
+    (void)hipGraphDestroy(pl->graph);
+    pl->gexec = nullptr;
+    pl->graph = nullptr;
+    pl->g_in = nullptr;
+    pl->g_out = nullptr;
+  }
+  if (st != FFS_OK) return st;
+  if (e != hipSuccess) return fail(FFS_ERR_HIP, "run_host failed: %s", hipGetErrorString(e));
+  return FFS_OK;
+}
+
+const char* ffs_last_error(void) { return g_err.c_str(); }
+const char* ffs_version(void) { return FFS_VERSION_STRING; }
+
+}  // extern "C"

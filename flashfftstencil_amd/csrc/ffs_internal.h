@@ -1,0 +1,50 @@
+// ffs_internal.h -- types shared by the host side (ffs_api.cpp) and the gfx950 kernels (ffs_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ffsk {
+
+// Tile families.  Every family runs its DFTs on 16x16 MFMA blocks.
+enum TileKind : int32_t {
+  KIND_2D16 = 0,   // 2D, 16x16 window
+  KIND_1DPFA = 1,  // 1D, 240-sample window as a 15x16 prime-factor (Good-Thomas) matrix
+  KIND_3D16 = 2,   // 3D, 16x16x16 window (one workgroup of 4 waves per tile pair)
+};
+
+// Operand-table layout in doubles (one table per plan, generated on the GPU in FP64):
+//   DFT B-operands  [axis 0..2][arr: R, S = Fr+Fi, D = Fi-Fr][chunk 0..3][lane 0..63]
+//   spectrum        [main|tail][...] laid out per tile family, see spectrum_len()
+constexpr int kLanes = 64;
+constexpr int kOpsPerAxis = 3 * 4 * kLanes;  // 768 doubles
+constexpr int kOpsLen = 3 * kOpsPerAxis;     // 2304 doubles
+
+// Same operands rounded for the mixed forms (float): [axis][arr][term hi/mid/lo][chunk][lane]
+constexpr int kOpsF32PerAxis = 3 * 3 * 4 * kLanes;
+
+inline int64_t spectrum_len(int kind) {  // complex values per spectrum (one depth)
+  return kind == KIND_3D16 ? 16 * 16 * 16 : 16 * 16;
+}
+
+struct TileArgs {
+  const double* src;
+  double* dst;
+  const double* ops;  // DFT operands (doubles)
+  const double* H;    // spectrum^t / (P0 P1 [P2]) for this block, family layout, [re plane | im plane]
+  const float* opsf;  // split float operands (mixed forms)
+  int64_t n0, n1, n2;       // grid sides (unused axes = 1); 1D uses n1
+  int64_t tiles0, tiles1, tiles2;
+  int64_t pairs;            // tile pairs in the block
+  int32_t V0, V1, V2;       // valid widths
+  int32_t h0, h1, h2;       // window start offset (valid region starts at h in the window)
+  int32_t precision;
+  int32_t pad_;
+};
+
+// Launch helpers implemented in ffs_kernels.hip.
+hipError_t gen_operands(double* ops, float* opsf, const int32_t* axis_len, hipStream_t s);
+hipError_t gen_spectrum(double* H, int kind, const int32_t* axis_len, const double* d_taps, int32_t ntaps_per_axis,
+                        int32_t dims, int32_t convention, int32_t radius, int64_t t, hipStream_t s);
+hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus);
+
+}  // namespace ffsk

@@ -1,0 +1,144 @@
+/*
+ * ffstencil.h -- C ABI of libffstencil, the MI355X (gfx950) tiled FFT-stencil engine.
+ *
+ * A periodic, constant-coefficient stencil of radius r (2r+1 taps per axis) is applied T times to a
+ * compact FP64 grid of 1, 2 or 3 dimensions.  The grid is cut into tiles; every tile window (valid
+ * region plus an r*t halo, gathered with periodic wrap) is transformed by DFT-as-GEMM on the matrix
+ * cores, multiplied by the stencil spectrum raised to the fused depth t, transformed back, and only
+ * the valid region is stored (overlap-save).  T steps run as floor(T/t) fused blocks plus one tail
+ * block, ping-ponged through HBM and replayed from a hipGraph.
+ *
+ * All pointers are plain host or device pointers; sizes are element counts; no HIP or torch types
+ * appear in the signatures (a stream is passed as `void*` holding a hipStream_t, NULL = null stream).
+ * Functions never exit the process: they return an ffs_status and leave a message for
+ * ffs_last_error() (thread-local).
+ *
+ * Reference interfaces replaced (KevinWu2017/FlashFFTStencil):
+ *   - the per-dimension `CreatePlan(double* k, int K, bool print)` that fills __constant__ DFT
+ *     matrices and the kernel spectrum (src/1D/create_fft_pfa_plan.cu:19, src/2D/create_fft_pfa_plan.cu:17,
+ *     src/3D/create_fft_pfa_plan.cu:17)                                    -> ffs_plan_create
+ *   - the time loop launching the tile kernel T (or T/t) times on the padded grid
+ *     (src/1D/1d_main.cu:164-177, src/2D/2d_main.cu:150-170, src/3D/3d_main.cu:147-170) -> ffs_run
+ *   - the shape table of the drivers (src/1D/1d_main.cu:44-65, src/2D/2d_main.cu:49-52,
+ *     src/3D/3d_main.cu:44-47) and their rand() weights (src/1D/1d_main.cu:130-141,
+ *     src/2D/2d_main.cu:113-126, src/3D/3d_main.cu:111-127)                 -> ffs_stencil_named
+ *   - the rand() grid initialisation (src/1D/1d_main.cu:115-129, src/2D/2d_main.cu:92-112,
+ *     src/3D/3d_main.cu:86-110)                                             -> ffs_fill_uniform
+ *   - the CSV report line (src/1D/1d_main.cu:187-195, src/2D/2d_main.cu:179-185,
+ *     src/3D/3d_main.cu:179-185)                                            -> ffs_format_csv
+ */
+#ifndef FFSTENCIL_H
+#define FFSTENCIL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFS_VERSION_STRING "0.1.0"
+
+typedef enum ffs_status {
+  FFS_OK = 0,
+  FFS_ERR_INVALID_ARG = 1,   /* bad shape, size, weights, T, pointer ... */
+  FFS_ERR_UNSUPPORTED = 2,   /* valid request this build does not implement (e.g. tile too small) */
+  FFS_ERR_OUT_OF_MEMORY = 3, /* device or host allocation failed */
+  FFS_ERR_HIP = 4,           /* a HIP runtime call failed */
+  FFS_ERR_NO_DEVICE = 5,     /* no gfx950 device visible */
+  FFS_ERR_INTERNAL = 6
+} ffs_status;
+
+/* Arithmetic of the DFT contractions.  The grid state between fused blocks, the DFT matrices and
+ * the spectrum power are FP64 in every form. */
+typedef enum ffs_precision {
+  FFS_PREC_FP64 = 0,       /* FP64 MFMA (v_mfma_f64_16x16x4f64) contractions: the drop-in form */
+  FFS_PREC_MIXED_BF16 = 1, /* 3-term BF16 split operands, FP32 accumulation (v_mfma_f32_16x16x32_bf16) */
+  FFS_PREC_MIXED_F32 = 2   /* FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4f32) */
+} ffs_precision;
+
+/* Index convention of one step.
+ *   REF:     out[p] = sum_{a in [0,2r]^d} w[a] * in[(p - a) mod n]   (true convolution anchored at the
+ *            first tap; the reference's CPU check composition, src/2D/check_correct.hpp:64-100 with
+ *            rotateKernel180 :17-31, and src/1D/1d_main.cu:220-223)
+ *   CENTRED: out[p] = sum_{o in [-r,r]^d} w[o + r] * in[(p + o) mod n]   (src/1D/check_correct.hpp:15-38) */
+typedef enum ffs_convention { FFS_CONV_REF = 0, FFS_CONV_CENTRED = 1 } ffs_convention;
+
+typedef struct ffs_problem {
+  int32_t dims;          /* 1, 2 or 3 */
+  int32_t radius;        /* r >= 0; taps per axis K = 2r+1 */
+  int64_t sizes[3];      /* grid sides, slowest axis first; entries past `dims` are ignored */
+  const double* weights; /* K^dims taps, row-major (slowest axis first); copied at plan creation */
+  int64_t steps;         /* T >= 0 */
+  int32_t precision;     /* ffs_precision */
+  int32_t convention;    /* ffs_convention */
+  int32_t tile;          /* DFT length per axis; 0 = automatic (2D/3D: 16; 1D: 240 = 16x15 prime-factor) */
+  int32_t fuse;          /* fused depth t; 0 = automatic */
+  int32_t use_graph;     /* 1 = replay the block loop from a hipGraph, 0 = plain launches */
+  int32_t device;        /* HIP device ordinal */
+  int32_t reserved[4];
+} ffs_problem;
+
+/* What a plan chose. */
+typedef struct ffs_plan_info {
+  int32_t dims, radius;
+  int64_t sizes[3];
+  int64_t steps;
+  int32_t precision, convention;
+  int32_t tile[3];        /* DFT length per axis (1 for unused axes) */
+  int32_t valid[3];       /* valid output width per tile and axis, V = P - 2 r t */
+  int32_t fuse;           /* t */
+  int32_t fuse_tail;      /* T mod t (0 = no tail block) */
+  int64_t blocks;         /* fused blocks per run, tail included */
+  int64_t tiles;          /* tiles per block */
+  int64_t launches;       /* kernel launches per run */
+  double flops_per_block; /* matrix-core flops one block executes (see header comment of the kernels) */
+  double bytes_per_block; /* algorithmic HBM bytes of one block: 16 B per grid point */
+} ffs_plan_info;
+
+typedef struct ffs_plan ffs_plan;
+
+/* Fill `p` with defaults: dims 2, radius 1, sizes 1, T 1, FP64, REF, automatic tile/fuse, graph on. */
+ffs_status ffs_problem_init(ffs_problem* p);
+
+/* Named stencils.  `shape` is one of Heat-1D, 1D5P, 1D7P, Heat-2D, Box2D9P, Star2D9P, Box2D25P,
+ * Heat-3D, Box3D27P (Box3d27P accepted).  Heat-1D is [0.25, 0.5, 0.25]; the others carry weights
+ * drawn uniform[0,1) from `seed` on the stencil's support and normalised to sum 1 (zeros off the
+ * support).  Writes dims, radius and (2r+1)^dims weights into `weights` (capacity in doubles).
+ * `tag` (capacity >= 16) receives the reference's CSV tag (1d1r, box_2d1r, ...); may be NULL. */
+ffs_status ffs_stencil_named(const char* shape, uint64_t seed, int32_t* dims, int32_t* radius,
+                             double* weights, int32_t capacity, char* tag, int32_t tag_capacity);
+
+/* Host helper: n doubles uniform in [0,1) from SplitMix64(seed), 53-bit mantissas. */
+ffs_status ffs_fill_uniform(uint64_t seed, double* host_out, int64_t n);
+
+/* Validate `p`, choose (P, t, V), allocate device tables and the ping-pong buffer, generate the DFT
+ * operands and the FP64 spectrum power on the GPU.  Validation happens before any HIP call. */
+ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out);
+ffs_status ffs_plan_get_info(const ffs_plan* plan, ffs_plan_info* info);
+ffs_status ffs_plan_destroy(ffs_plan* plan);
+
+/* Advance T steps: d_out = S^T(d_in).  Device pointers to compact row-major FP64 grids of the plan's
+ * sizes; d_in is not modified and must not alias d_out.  Asynchronous on `stream`. */
+ffs_status ffs_run(ffs_plan* plan, const double* d_in, double* d_out, void* stream);
+
+/* Same as ffs_run on host buffers (allocates, copies, runs, synchronises). */
+ffs_status ffs_run_host(ffs_plan* plan, const double* h_in, double* h_out);
+
+/* One fused block only (depth t, or the tail depth when `tail` != 0 and the plan has a tail),
+ * d_out = S^t(d_in).  One kernel launch; used to time the tile kernel alone. */
+ffs_status ffs_run_block(ffs_plan* plan, const double* d_in, double* d_out, int32_t tail, void* stream);
+
+/* Reference-format report line:
+ *   "FlashFFTStencil, <tag>, <d1>, <d2>, <d3>, <T>, <ms>, <GStencil/s>"
+ * with leading 1s for unused dimensions and GStencil/s = points * T / ms / 1e6. */
+ffs_status ffs_format_csv(const char* tag, int32_t dims, const int64_t* sizes, int64_t steps, double ms,
+                          char* buf, int32_t capacity);
+
+const char* ffs_last_error(void);
+const char* ffs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FFSTENCIL_H */

@@ -1,0 +1,125 @@
+"""GPU parity of the FP64 form against the FP64 direct-sweep oracle, through the C ABI.
+
+Bar: max|gpu - oracle| / max|oracle| <= 1e-12 for the FP64 form (the stated drop-in tolerance).
+Cases follow the reference's shapes and its boundary convention (periodic, ref index convention),
+plus the cases the reference cannot run: sizes not divisible by the valid width, grids smaller than a
+tile, odd tile counts, T not a multiple of the fused depth, T = 0.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+
+
+def _run(ffs, torch, x, radius, w, steps, **kw):
+    plan = ffs.Plan(x.ndim, x.shape, w, radius, steps, **kw)
+    d_in = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    d_out = torch.empty_like(d_in)
+    plan.run(d_in, d_out)
+    torch.cuda.synchronize()
+    info = plan.info
+    plan.close()
+    return d_out.cpu().numpy(), info
+
+
+SHAPES_2D = ["Heat-2D", "Box2D9P", "Star2D9P", "Box2D25P"]
+
+
+@pytest.mark.parametrize("shape", SHAPES_2D)
+@pytest.mark.parametrize("size", [(37, 53), (16, 16), (5, 7), (64, 40), (1, 19)])
+@pytest.mark.parametrize("steps", [1, 2, 5])
+def test_2d_fp64_named(ffs, oracle, torch_cuda, shape, size, steps):
+    dims, r, w, _ = ffs.named_stencil(shape, seed=7)
+    x = oracle.uniform(1234 + size[0] * 31 + size[1], size)
+    got, info = _run(ffs, torch_cuda, x, r, w, steps)
+    ref = oracle.sweep(x, r, w, steps)
+    _, rel, _ = oracle.errors(got, ref)
+    print(shape, size, steps, info["fuse"], info["fuse_tail"], rel)
+    assert rel <= TOL_FP64, (shape, size, steps, info, rel)
+
+
+@pytest.mark.parametrize("convention", ["ref", "centred"])
+@pytest.mark.parametrize("fuse", [1, 2, 3])
+def test_2d_fp64_fuse_and_convention(ffs, oracle, torch_cuda, convention, fuse):
+    rng = np.random.default_rng(fuse)
+    w = rng.random(9)
+    w /= w.sum()
+    x = oracle.uniform(99, (45, 61))
+    conv = 0 if convention == "ref" else 1
+    for steps in (fuse, 2 * fuse + 1, 7):
+        got, info = _run(ffs, torch_cuda, x, 1, w, steps, convention=convention, fuse=fuse)
+        ref = oracle.sweep(x, 1, w, steps, convention=conv)
+        _, rel, _ = oracle.errors(got, ref)
+        assert rel <= TOL_FP64, (convention, fuse, steps, info, rel)
+
+
+def test_2d_unnormalised_weights(ffs, oracle, torch_cuda):
+    # weights that neither sum to 1 nor are positive: the spectrum power must still be exact
+    rng = np.random.default_rng(3)
+    w = rng.standard_normal(25) * 0.3
+    x = oracle.uniform(5, (50, 50))
+    got, _ = _run(ffs, torch_cuda, x, 2, w, 3)
+    ref = oracle.sweep(x, 2, w, 3)
+    _, rel, _ = oracle.errors(got, ref)
+    assert rel <= TOL_FP64
+
+
+@pytest.mark.parametrize("shape", ["Heat-1D", "1D5P", "1D7P"])
+@pytest.mark.parametrize("n", [1, 7, 240, 241, 1000, 4099])
+@pytest.mark.parametrize("steps", [1, 13, 64])
+def test_1d_fp64_named(ffs, oracle, torch_cuda, shape, n, steps):
+    dims, r, w, _ = ffs.named_stencil(shape, seed=11)
+    x = oracle.uniform(777 + n, (n,))
+    got, info = _run(ffs, torch_cuda, x, r, w, steps)
+    ref = oracle.sweep(x, r, w, steps)
+    _, rel, _ = oracle.errors(got, ref)
+    assert rel <= TOL_FP64, (shape, n, steps, info, rel)
+
+
+def test_1d_centred_and_forced_fuse(ffs, oracle, torch_cuda):
+    rng = np.random.default_rng(8)
+    w = rng.random(5)
+    x = oracle.uniform(17, (3001,))
+    for fuse in (1, 7, 50):
+        for conv in (0, 1):
+            got, info = _run(ffs, torch_cuda, x, 2, w / w.sum(), 101, convention=conv, fuse=fuse)
+            ref = oracle.sweep(x, 2, w / w.sum(), 101, convention=conv)
+            _, rel, _ = oracle.errors(got, ref)
+            assert rel <= TOL_FP64, (fuse, conv, info, rel)
+
+
+def test_zero_steps_is_copy(ffs, oracle, torch_cuda):
+    _, r, w, _ = ffs.named_stencil("Box2D9P")
+    x = oracle.uniform(1, (33, 17))
+    got, _ = _run(ffs, torch_cuda, x, r, w, 0)
+    assert np.array_equal(got, x)
+
+
+def test_deterministic_and_graph_equals_plain(ffs, oracle, torch_cuda):
+    torch = torch_cuda
+    _, r, w, _ = ffs.named_stencil("Box2D25P", seed=3)
+    x = oracle.uniform(2, (200, 150))
+    d_in = torch.from_numpy(x).cuda()
+    outs = []
+    for use_graph in (True, True, False):
+        plan = ffs.Plan(2, x.shape, w, r, 9, use_graph=use_graph)
+        d_out = torch.empty_like(d_in)
+        plan.run(d_in, d_out)
+        plan.run(d_in, d_out)  # replay
+        torch.cuda.synchronize()
+        outs.append(d_out.cpu().numpy())
+        plan.close()
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])
+
+
+def test_run_host_matches_device(ffs, oracle, torch_cuda):
+    _, r, w, _ = ffs.named_stencil("Heat-2D", seed=1)
+    x = oracle.uniform(3, (70, 30))
+    plan = ffs.Plan(2, x.shape, w, r, 6)
+    got = plan.run_host(x)
+    plan.close()
+    ref = oracle.sweep(x, r, w, 6)
+    assert oracle.errors(got, ref)[1] <= TOL_FP64
