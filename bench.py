@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""bench.py -- GStencils/s of the tiled FFT-stencil path on one MI355X.
+
+One bench "step" is one full pass of the hot path: T stencil steps over the whole synthetic grid
+(ffs_run: floor(T/t) fused blocks + tail, replayed from a hipGraph), input already resident in HBM.
+Default workload: Box2D25P, 8192 x 8192, T = 256, weights uniform[0,1) normalised to sum 1, grid
+uniform[0,1), both from SplitMix64 seeded 0x5EED0002.  `value` is the mixed-precision form (BF16
+split operands, FP32 accumulation, FP64 state and spectrum power); the plain FP64-MFMA form is timed
+beside it and both are checked against the FP64 direct sweep (the oracle) on a bounded sample.
+
+Printed: one JSON line (see the keys below).  --gpus N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAKS = {  # MI355X published dense peaks
+    "fp64": ("mfma", 78.6e12, "TFLOP/s"),
+    "bf16": ("mfma", 2.5e15, "TFLOP/s"),
+    "f32": ("mfma", 157.3e12, "TFLOP/s"),
+}
+HBM_PEAK = 8.0e12
+
+WORKLOADS = {
+    "box2d25p": dict(shape="Box2D25P", sizes=(8192, 8192), T=256, seed=0x5EED0002, forms=("bf16", "fp64")),
+    "box2d9p": dict(shape="Box2D9P", sizes=(8192, 8192), T=128, seed=0x5EED0001, forms=("fp64",)),
+    "heat1d": dict(shape="Heat-1D", sizes=(1048576,), T=64, seed=0x5EED0000, forms=("fp64",)),
+    "box3d27p": dict(shape="Box3D27P", sizes=(512, 512, 512), T=32, seed=0x5EED0003, forms=("f32", "fp64")),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed full runs")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="box2d25p", choices=sorted(WORKLOADS))
+    ap.add_argument("--kernel-reps", type=int, default=10, help="timed single-block launches for the roofline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle sample check")
+    ap.add_argument("--forms", default=None, help="comma list overriding the workload's precision forms")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import flashfftstencil_amd as ffs
+
+    torch.cuda.set_device(0)
+    wl = WORKLOADS[args.workload]
+    forms = tuple(args.forms.split(",")) if args.forms else wl["forms"]
+    _, radius, w, tag = ffs.named_stencil(wl["shape"], seed=wl["seed"])
+    sizes, T = wl["sizes"], wl["T"]
+    npts = int(np.prod(sizes))
+    x = ffs.fill_uniform(wl["seed"] + 1, npts).reshape(sizes)
+    d_in = torch.from_numpy(x).cuda()
+    stream = torch.cuda.current_stream()
+
+    results = {}
+    outputs = {}
+    for form in forms:
+        try:
+            plan = ffs.Plan(len(sizes), sizes, w, radius, T, precision=form)
+        except ffs.FFSError as e:
+            log(f"form {form} unavailable: {e}")
+            continue
+        info = plan.info
+        d_out = torch.empty_like(d_in)
+        for _ in range(args.warmup):
+            plan.run(d_in, d_out)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            plan.run(d_in, d_out)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = e0.elapsed_time(e1) / args.steps
+        gst = npts * T / (ms * 1e-3) / 1e9
+        # dominant kernel alone: one fused block per launch, timed with events on the launch stream
+        d_tmp = torch.empty_like(d_in)
+        plan.run_block(d_in, d_tmp)
+        torch.cuda.synchronize()
+        k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        k0.record(stream)
+        for _ in range(args.kernel_reps):
+            plan.run_block(d_in, d_tmp)
+        k1.record(stream)
+        torch.cuda.synchronize()
+        kms = k0.elapsed_time(k1) / args.kernel_reps
+        del d_tmp
+        bound, peak, unit = PEAKS[form]
+        achieved = info["flops_per_block"] / (kms * 1e-3)
+        results[form] = dict(
+            value=gst, ms_per_step=ms, wall_s=wall, kernel_ms=kms, info=info,
+            roofline={"bound": bound, "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": unit,
+                      "frac": achieved / peak, "traffic": None,
+                      "hbm_algorithmic_GBps": info["bytes_per_block"] / (kms * 1e-3) / 1e9},
+        )
+        outputs[form] = d_out
+        log(f"{form}: {gst:.1f} GStencil/s, {ms:.2f} ms/run, block kernel {kms:.3f} ms, "
+            f"{achieved / 1e12:.2f} TFLOP/s ({achieved / peak:.1%} of {form} MFMA peak), P={info['tile']} "
+            f"t={info['fuse']} tail={info['fuse_tail']} V={info['valid']} blocks={info['blocks']}")
+        plan.close()
+
+    if not results:
+        print("bench.py: no precision form could be planned", file=sys.stderr)
+        sys.exit(1)
+
+    errors = {}
+    if "fp64" in outputs:
+        ref64 = outputs["fp64"]
+        for form, d in outputs.items():
+            if form == "fp64":
+                continue
+            diff = (d - ref64).abs()
+            errors[f"{form}_vs_fp64_full_T"] = {
+                "max_rel": float(diff.max() / ref64.abs().max()),
+                "rel_l2": float(torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(ref64)),
+            }
+
+    cpu = None
+    if not args.no_cpu:
+        from tests import _oracle
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        t1 = time.perf_counter()
+        _oracle.sweep(x, radius, w, 1, threads=threads)
+        one = time.perf_counter() - t1
+        ts = int(max(1, min(T, math.floor(args.cpu_seconds / max(one, 1e-6)))))
+        t1 = time.perf_counter()
+        ref = _oracle.sweep(x, radius, w, ts, threads=threads)
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": npts * ts / cpu_s / 1e9, "unit": "GStencils/s", "cores": threads, "kind": "port",
+               "sample": f"{wl['shape']} {'x'.join(map(str, sizes))}, {ts} of the {T} steps, FP64 direct periodic "
+                         f"sweep (oracle/oracle.c, OpenMP {threads} threads), {cpu_s:.1f} s"}
+        log(f"cpu baseline: {cpu['value']:.3f} GStencil/s on {threads} threads ({ts} steps, {cpu_s:.1f} s)")
+        for form in outputs:
+            plan = ffs.Plan(len(sizes), sizes, w, radius, ts, precision=form)
+            d = torch.empty_like(d_in)
+            plan.run(d_in, d)
+            torch.cuda.synchronize()
+            got = d.cpu().numpy()
+            plan.close()
+            _, rel, l2 = _oracle.errors(got, ref)
+            errors[f"{form}_vs_oracle_T{ts}"] = {"max_rel": rel, "rel_l2": l2}
+            log(f"{form} vs oracle after {ts} steps: max-rel {rel:.3e} rel-L2 {l2:.3e}")
+
+    main_form = forms[0] if forms[0] in results else next(iter(results))
+    r = results[main_form]
+    traffic = _traffic(args.workload, main_form)
+    if traffic is not None:
+        r["roofline"]["traffic"] = traffic
+    line = {
+        "metric": "GStencils/s (points x steps / s)",
+        "value": r["value"],
+        "unit": "GStencils/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "bf16": "bf16x3-split/f32-acc (f64 state)", "f32": "f32 (f64 state)"}[main_form],
+        "data": "synthetic (SplitMix64 uniform[0,1) grid and weights)",
+        "config": {
+            "workload": f"{wl['shape']} {'x'.join(map(str, sizes))} T={T}",
+            "stencil": wl["shape"], "grid": list(sizes), "T": T, "form": main_form,
+            "tile": r["info"]["tile"], "fuse": r["info"]["fuse"], "fuse_tail": r["info"]["fuse_tail"],
+            "valid": r["info"]["valid"], "blocks": r["info"]["blocks"],
+        },
+        "roofline": r["roofline"],
+        "cpu_baseline": cpu,
+        "forms": {f: {"value": v["value"], "ms_per_step": v["ms_per_step"], "kernel_ms": v["kernel_ms"],
+                      "roofline": v["roofline"], "fuse": v["info"]["fuse"]} for f, v in results.items()},
+        "errors": errors,
+    }
+    if "fp64" in results and main_form != "fp64":
+        line["speedup_vs_fp64"] = r["value"] / results["fp64"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+def _traffic(workload, form):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data.get(f"{workload}/{form}")
+    except (OSError, ValueError):
+        return None
+
+
+if __name__ == "__main__":
+    main()

@@ -91,7 +91,9 @@ __device__ __forceinline__ void to_chunks(const d4& v, double (&c)[4]) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// 2D 16x16 and 1D 240-sample prime-factor tiles, FP64.  One wave per tile pair, grid-stride loop.
+// 2D 16x16 and 1D 240-sample prime-factor tiles, FP64.  One wave per tile pair, grid-stride loop
+// over tile pairs with the next pair's window prefetched into registers while the current pair is
+// on the matrix cores.  Tile indexing is wave-uniform (scalar registers).
 //
 // 1D prime-factor map (Good-Thomas, 240 = 15 x 16 coprime; the reference uses 56 = 8 x 7,
 // src/1D/rfftstencil_pfa/rfft_8_pfa_fastcomplex.cu:39-47 and src/1D/create_fft_pfa_plan.cu:171-178):
@@ -99,16 +101,123 @@ __device__ __forceinline__ void to_chunks(const d4& v, double (&c)[4]) {
 //   spectrum index k = (15 k1 + 16 k0) mod 240 for (row freq k0 < 15, col freq k1)
 // Row 15 of the 16x16 block is padding: loaded as zero, its DFT rows/cols are zero, never stored.
 // ---------------------------------------------------------------------------------------------
+struct PairCoord {
+  int64_t r0;     // 2D: first output row of the pair (both tiles share it)
+  int64_t orgA;   // 2D: first output column of tile A; 1D: first output sample of tile A
+  int64_t orgB;
+  bool hasB;
+};
+
+template <int KIND>
+__device__ __forceinline__ PairCoord pair_coord(const TileArgs& a, int64_t tp) {
+  PairCoord pc;
+  if (KIND == KIND_2D16) {
+    const int64_t pairs1 = (a.tiles1 + 1) >> 1;
+    const int64_t T0 = tp / pairs1, T1 = (tp - T0 * pairs1) * 2;
+    pc.hasB = (T1 + 1) < a.tiles1;
+    pc.r0 = T0 * a.V0;
+    pc.orgA = T1 * a.V1;
+  } else {
+    const int64_t T1 = tp * 2;
+    pc.hasB = (T1 + 1) < a.tiles1;
+    pc.r0 = 0;
+    pc.orgA = T1 * a.V1;
+  }
+  pc.orgB = pc.orgA + a.V1;
+  return pc;
+}
+
+// Load the pair's windows as stage-1 A operands: chunk c of lane (j + 16q) = window[4c + q][j].
+template <int KIND>
+__device__ __forceinline__ void gather_pair(const TileArgs& a, const PairCoord& pc, int q, int j, double (&xr)[4],
+                                            double (&xi)[4]) {
+  const double* __restrict__ src = a.src;
+  if (KIND == KIND_2D16) {
+    const int64_t rw = pc.r0 - a.h0, cwA = pc.orgA - a.h1, cwB = pc.orgB - a.h1;
+    const bool interior = rw >= 0 && rw + 16 <= a.n0 && cwA >= 0 && (pc.hasB ? cwB : cwA) + 16 <= a.n1;
+    if (interior) {  // wave-uniform: plain affine addressing
+      const double* p = src + (rw + q) * a.n1 + cwA + j;
+      const int64_t step = 4 * a.n1;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        xr[c] = p[c * step];
+        xi[c] = pc.hasB ? p[c * step + a.V1] : 0.0;
+      }
+    } else {
+      const int64_t cA = wrap_near(cwA + j, a.n1);
+      const int64_t cB = wrap_near(cwB + j, a.n1);
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int64_t row = wrap_near(rw + 4 * c + q, a.n0) * a.n1;
+        xr[c] = src[row + cA];
+        xi[c] = pc.hasB ? src[row + cB] : 0.0;
+      }
+    }
+  } else {
+    const int64_t wA = pc.orgA - a.h1, wB = pc.orgB - a.h1;
+    const bool interior = wA >= 0 && (pc.hasB ? wB : wA) + 240 <= a.n1;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int row = 4 * c + q;
+      const int n = (225 * j + 16 * row) % 240;
+      const bool pad = row >= 15;
+      if (interior) {
+        xr[c] = pad ? 0.0 : src[wA + n];
+        xi[c] = (pad || !pc.hasB) ? 0.0 : src[wB + n];
+      } else {
+        xr[c] = pad ? 0.0 : src[wrap_near(wA + n, a.n1)];
+        xi[c] = (pad || !pc.hasB) ? 0.0 : src[wrap_near(wB + n, a.n1)];
+      }
+    }
+  }
+}
+
+template <int KIND>
+__device__ __forceinline__ void store_pair(const TileArgs& a, const PairCoord& pc, int q, int j, const d4& yr,
+                                           const d4& yi) {
+  if (KIND == KIND_2D16) {
+    const int64_t cA = pc.orgA + j - a.h1, cB = pc.orgB + j - a.h1;
+    const bool colok = j >= a.h1 && j < a.h1 + a.V1;
+    const bool okA = colok && cA < a.n1;
+    const bool okB = colok && pc.hasB && cB < a.n1;
+    double* base = a.dst + (pc.r0 + q - a.h0) * a.n1;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = q + 4 * v;
+      if (row >= a.h0 && row < a.h0 + a.V0 && pc.r0 + row - a.h0 < a.n0) {
+        double* drow = base + (int64_t)(4 * v) * a.n1;
+        if (okA) drow[cA] = yr[v];
+        if (okB) drow[cB] = yi[v];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = q + 4 * v;
+      if (row < 15) {
+        const int n = (225 * j + 16 * row) % 240;
+        if (n >= a.h1 && n < a.h1 + a.V1) {
+          const int64_t pA = pc.orgA + n - a.h1, pB = pc.orgB + n - a.h1;
+          if (pA < a.n1) a.dst[pA] = yr[v];
+          if (pc.hasB && pB < a.n1) a.dst[pB] = yi[v];
+        }
+      }
+    }
+  }
+}
+
 template <int KIND>
 __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
-  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
+  // 2D: both axes are 16-point DFTs and share one operand set; 1D: 15-point rows, 16-point columns.
   OpsAxis f0, f1;
   load_ops(f0, a.ops, 0, lane);
-  load_ops(f1, a.ops, 1, lane);
+  if (KIND != KIND_2D16) load_ops(f1, a.ops, 1, lane);
+  const OpsAxis& g1 = (KIND == KIND_2D16) ? f0 : f1;
   d4 hr, hi;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
@@ -116,39 +225,19 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
     hi[v] = a.H[256 + v * kLanes + lane];
   }
 
-  for (int64_t tp = gw; tp < a.pairs; tp += W) {
-    double xr[4], xi[4];
-    int64_t orgA, orgB;  // tile origins: 2D -> (row origin, col origin) packed below; 1D -> sample origin
-    bool hasB;
-    int64_t r0 = 0;
-    if (KIND == KIND_2D16) {
-      const int64_t pairs1 = (a.tiles1 + 1) >> 1;
-      const int64_t T0 = tp / pairs1, T1 = (tp - T0 * pairs1) * 2;
-      hasB = (T1 + 1) < a.tiles1;
-      r0 = T0 * a.V0;
-      orgA = T1 * a.V1;
-      orgB = orgA + a.V1;
-      const int64_t cA = wrap_near(orgA - a.h1 + j, a.n1);
-      const int64_t cB = wrap_near(orgB - a.h1 + j, a.n1);
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const int64_t row = wrap_near(r0 - a.h0 + 4 * c + q, a.n0) * a.n1;
-        xr[c] = a.src[row + cA];
-        xi[c] = hasB ? a.src[row + cB] : 0.0;
-      }
-    } else {
-      const int64_t T1 = tp * 2;
-      hasB = (T1 + 1) < a.tiles1;
-      orgA = T1 * a.V1;
-      orgB = orgA + a.V1;
-#pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const int row = 4 * c + q;
-        const int n = (225 * j + 16 * row) % 240;
-        const bool pad = row >= 15;
-        xr[c] = pad ? 0.0 : a.src[wrap_near(orgA - a.h1 + n, a.n1)];
-        xi[c] = (pad || !hasB) ? 0.0 : a.src[wrap_near(orgB - a.h1 + n, a.n1)];
-      }
+  int64_t tp = gw;
+  if (tp >= a.pairs) return;
+  PairCoord pc = pair_coord<KIND>(a, tp);
+  double xr[4], xi[4];
+  gather_pair<KIND>(a, pc, q, j, xr, xi);
+  for (; tp < a.pairs; tp += W) {
+    // prefetch the next pair while this one computes
+    const int64_t tn = tp + W;
+    PairCoord pn = pc;
+    double nr[4] = {0.0, 0.0, 0.0, 0.0}, ni[4] = {0.0, 0.0, 0.0, 0.0};
+    if (tn < a.pairs) {
+      pn = pair_coord<KIND>(a, tn);
+      gather_pair<KIND>(a, pn, q, j, nr, ni);
     }
 
     d4 sr, si;
@@ -156,7 +245,7 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
     double cr[4], ci[4];
     to_chunks(sr, cr);
     to_chunks(si, ci);
-    cstage<false>(cr, ci, f1, sr, si);  // contract axis 1 -> Y[k0 = q+4v][k1 = j]
+    cstage<false>(cr, ci, g1, sr, si);  // contract axis 1 -> Y[k0 = q+4v][k1 = j]
     {
       const d4 yr = sr, yi = si;
       sr = yr * hr - yi * hi;
@@ -167,36 +256,14 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
     cstage<true>(cr, ci, f0, sr, si);  // inverse over k0
     to_chunks(sr, cr);
     to_chunks(si, ci);
-    cstage<true>(cr, ci, f1, sr, si);  // inverse over k1 -> X'[n0 = q+4v][n1 = j]
+    cstage<true>(cr, ci, g1, sr, si);  // inverse over k1 -> X'[n0 = q+4v][n1 = j]
+    store_pair<KIND>(a, pc, q, j, sr, si);
 
-    if (KIND == KIND_2D16) {
-      const int64_t cA = orgA + j - a.h1, cB = orgB + j - a.h1;
-      const bool colok = j >= a.h1 && j < a.h1 + a.V1;
-      const bool okA = colok && cA < a.n1;
-      const bool okB = colok && hasB && cB < a.n1;
+    pc = pn;
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int row = q + 4 * v;
-        const int64_t y = r0 + row - a.h0;
-        if (row >= a.h0 && row < a.h0 + a.V0 && y < a.n0) {
-          double* drow = a.dst + y * a.n1;
-          if (okA) drow[cA] = sr[v];
-          if (okB) drow[cB] = si[v];
-        }
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int row = q + 4 * v;
-        if (row < 15) {
-          const int n = (225 * j + 16 * row) % 240;
-          if (n >= a.h1 && n < a.h1 + a.V1) {
-            const int64_t pA = orgA + n - a.h1, pB = orgB + n - a.h1;
-            if (pA < a.n1) a.dst[pA] = sr[v];
-            if (hasB && pB < a.n1) a.dst[pB] = si[v];
-          }
-        }
-      }
+    for (int c = 0; c < 4; c++) {
+      xr[c] = nr[c];
+      xi[c] = ni[c];
     }
   }
 }
