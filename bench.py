@@ -26,6 +26,7 @@ PEAKS = {  # MI355X published dense peaks
     "fp64": ("mfma", 78.6e12, "TFLOP/s"),
     "bf16": ("mfma", 2.5e15, "TFLOP/s"),
     "f32": ("mfma", 157.3e12, "TFLOP/s"),
+    "bf16x2": ("mfma", 2.5e15, "TFLOP/s"),
 }
 HBM_PEAK = 8.0e12
 
@@ -179,7 +180,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "bf16": "bf16x3-split/f32-acc (f64 state)", "f32": "f32 (f64 state)"}[main_form],
+        "dtype": {"fp64": "f64", "bf16": "bf16x3-split/f32-acc (f64 state)", "bf16x2": "bf16x2-split/f32-acc (f64 state)",
+                  "f32": "f32 (f64 state)"}[main_form],
         "data": "synthetic (SplitMix64 uniform[0,1) grid and weights)",
         "config": {
             "workload": f"{wl['shape']} {'x'.join(map(str, sizes))} T={T}",

@@ -75,7 +75,9 @@ struct ffs_plan {
   int64_t pairs = 0;
   int32_t h_main[3] = {0, 0, 0}, h_tail[3] = {0, 0, 0};
   double* d_ops = nullptr;
+  uint32_t* d_opsm = nullptr;  // mixed-form operand table
   double* d_H = nullptr;       // [main | tail], each 2*len doubles
+  double dc_main = 1.0, dc_tail = 1.0;  // K(0)^t and K(0)^t_tail
   double* d_taps = nullptr;
   double* d_scratch = nullptr;  // ping-pong grid
   int64_t npoints = 0;
@@ -95,7 +97,7 @@ ffs_status validate(const ffs_problem* p, int64_t* npoints) {
   if (p->radius < 0) return fail(FFS_ERR_INVALID_ARG, "radius must be >= 0 (got %d)", p->radius);
   if (p->steps < 0) return fail(FFS_ERR_INVALID_ARG, "steps must be >= 0 (got %lld)", (long long)p->steps);
   if (!p->weights) return fail(FFS_ERR_INVALID_ARG, "weights pointer is NULL");
-  if (p->precision < FFS_PREC_FP64 || p->precision > FFS_PREC_MIXED_F32)
+  if (p->precision < FFS_PREC_FP64 || p->precision > FFS_PREC_MIXED_BF16X2)
     return fail(FFS_ERR_INVALID_ARG, "unknown precision %d", p->precision);
   if (p->convention != FFS_CONV_REF && p->convention != FFS_CONV_CENTRED)
     return fail(FFS_ERR_INVALID_ARG, "unknown convention %d", p->convention);
@@ -145,13 +147,19 @@ ffs_status choose_config(ffs_plan* pl) {
                   (long long)tmax);
     tbest = p.fuse;
   } else {
-    // maximise valid point-steps per tile: V^d * t (the per-tile DFT work is fixed)
+    // Maximise valid point-steps per unit time of one tile pair: 2 V^d t / max(compute, hbm), where the
+    // compute cost of a pair is fixed by the tile family and precision (cycles per SIMD, measured on
+    // MI355X) and the HBM cost is 16 B per valid point at ~3.26 B/cycle/SIMD (8 TB/s over 1024 SIMDs
+    // at 2.4 GHz).
+    const int dd = p.dims == 1 ? 1 : p.dims;
+    const double compute = p.precision == FFS_PREC_FP64 ? 4700.0 : (p.precision == FFS_PREC_MIXED_F32 ? 2600.0 : 1100.0);
+    const double hbm_per_point = 2.0 * 16.0 / 3.26;
     double best = -1;
     tbest = 1;
     const int64_t tcap = std::min<int64_t>(tmax, 4096);
     for (int64_t t = 1; t <= tcap; t++) {
-      const double Vv = Pmain - 2.0 * r * t;
-      const double score = std::pow(Vv, p.dims == 1 ? 1 : p.dims) * t;
+      const double Vd = std::pow(Pmain - 2.0 * r * t, dd);
+      const double score = Vd * t / std::max(compute * (p.dims == 3 ? 16.0 : 1.0), hbm_per_point * Vd);
       if (score > best * (1 + 1e-9)) {
         best = score;
         tbest = t;
@@ -201,6 +209,7 @@ void release(ffs_plan* pl) {
   if (pl->graph) (void)hipGraphDestroy(pl->graph);
   if (pl->cap_stream) (void)hipStreamDestroy(pl->cap_stream);
   if (pl->d_ops) (void)hipFree(pl->d_ops);
+  if (pl->d_opsm) (void)hipFree(pl->d_opsm);
   if (pl->d_H) (void)hipFree(pl->d_H);
   if (pl->d_taps) (void)hipFree(pl->d_taps);
   if (pl->d_scratch) (void)hipFree(pl->d_scratch);
@@ -213,7 +222,8 @@ ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, boo
   a.dst = dst;
   a.ops = pl->d_ops;
   a.H = pl->d_H + (tail ? 2 * pl->hlen : 0);
-  a.opsf = nullptr;
+  a.opsm = pl->d_opsm;
+  a.dc_gain = tail ? pl->dc_tail : pl->dc_main;
   const ffs_problem& p = pl->prob;
   if (p.dims == 1) {
     a.n0 = 1;
@@ -358,10 +368,6 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
     release(pl);
     return st;
   }
-  if (pl->prob.precision != FFS_PREC_FP64) {
-    release(pl);
-    return fail(FFS_ERR_UNSUPPORTED, "mixed precision forms are not built yet");
-  }
   // ---- device work starts here ----
   int ndev = 0;
   const hipError_t ce = hipGetDeviceCount(&ndev);
@@ -390,6 +396,7 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
     if (x != hipSuccess && e == hipSuccess) e = x;
   };
   chk(hipMalloc(&pl->d_ops, sizeof(double) * ffsk::kOpsLen));
+  if (p->precision != FFS_PREC_FP64) chk(hipMalloc(&pl->d_opsm, sizeof(uint32_t) * ffsk::kOpsMixedLen));
   chk(hipMalloc(&pl->d_H, sizeof(double) * 4 * pl->hlen));
   chk(hipMalloc(&pl->d_taps, sizeof(double) * ntaps));
   if (pl->nblocks >= 2) chk(hipMalloc(&pl->d_scratch, sizeof(double) * npoints));
@@ -399,10 +406,24 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
                 hipGetErrorString(e));
   }
   chk(hipMemcpy(pl->d_taps, pl->weights.data(), sizeof(double) * ntaps, hipMemcpyHostToDevice));
-  chk(ffsk::gen_operands(pl->d_ops, nullptr, pl->L, nullptr));
-  chk(ffsk::gen_spectrum(pl->d_H, pl->kind, pl->L, pl->d_taps, K, p->dims, p->convention, p->radius, pl->t, nullptr));
-  chk(ffsk::gen_spectrum(pl->d_H + 2 * pl->hlen, pl->kind, pl->L, pl->d_taps, K, p->dims, p->convention, p->radius,
-                        pl->t_tail > 0 ? pl->t_tail : pl->t, nullptr));
+  const int layout = p->precision == FFS_PREC_FP64 ? 0 : 1;
+  chk(ffsk::gen_operands(pl->d_ops, pl->d_opsm, pl->L, nullptr));
+  chk(ffsk::gen_spectrum(pl->d_H, pl->kind, layout, pl->L, pl->d_taps, K, p->dims, p->convention, p->radius, pl->t,
+                         nullptr));
+  chk(ffsk::gen_spectrum(pl->d_H + 2 * pl->hlen, pl->kind, layout, pl->L, pl->d_taps, K, p->dims, p->convention,
+                         p->radius, pl->t_tail > 0 ? pl->t_tail : pl->t, nullptr));
+  {  // gain of the constant mode, K(0)^t = (sum w)^t, by repeated squaring in FP64
+    double sw = 0.0;
+    for (double wv : pl->weights) sw += wv;
+    auto ipow = [](double b, int64_t e) {
+      double r = 1.0;
+      for (; e > 0; e >>= 1, b *= b)
+        if (e & 1) r *= b;
+      return r;
+    };
+    pl->dc_main = ipow(sw, pl->t);
+    pl->dc_tail = ipow(sw, pl->t_tail > 0 ? pl->t_tail : pl->t);
+  }
   chk(hipDeviceSynchronize());
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&pl->cap_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -441,10 +462,18 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   info->fuse_tail = (int32_t)pl->t_tail;
   info->blocks = pl->nblocks;
   info->launches = pl->nblocks;
-  // MFMA flops per tile pair: stages x 3 real GEMMs (Gauss form) x 2*16^3 per 16x16x16 GEMM x 16^(d-2) slices
+  // Matrix-core flops executed per tile pair: stages (2 per axis) x 16^(d-2) slices x real 16x16x16
+  // GEMMs per complex product x MFMA flops per real GEMM.
+  //   FP64: Gauss form, 3 GEMMs of 4 x v_mfma_f64_16x16x4f64 (2*16^3 flops each GEMM)
+  //   BF16: 4 GEMMs of NTERMS x v_mfma_f32_16x16x32_bf16 (2*16*16*32 flops each instruction)
+  //   FP32: 4 GEMMs of 4 x v_mfma_f32_16x16x4f32 (2*16^3 flops each GEMM)
   const double gemm = 2.0 * 16 * 16 * 16;
   const int dd = p.dims == 1 ? 2 : p.dims;
-  const double per_pair = 2.0 * dd * 3.0 * gemm * std::pow(16.0, dd - 2);
+  double per_cplx = 3.0 * gemm;
+  if (p.precision == FFS_PREC_MIXED_BF16) per_cplx = 4.0 * 3.0 * 2.0 * 16 * 16 * 32;
+  if (p.precision == FFS_PREC_MIXED_BF16X2) per_cplx = 4.0 * 2.0 * 2.0 * 16 * 16 * 32;
+  if (p.precision == FFS_PREC_MIXED_F32) per_cplx = 4.0 * gemm;
+  const double per_pair = 2.0 * dd * per_cplx * std::pow(16.0, dd - 2);
   info->flops_per_block = per_pair * (double)pl->pairs;
   info->bytes_per_block = 16.0 * (double)pl->npoints;
   return FFS_OK;

@@ -19,8 +19,17 @@ constexpr int kLanes = 64;
 constexpr int kOpsPerAxis = 3 * 4 * kLanes;  // 768 doubles
 constexpr int kOpsLen = 3 * kOpsPerAxis;     // 2304 doubles
 
-// Same operands rounded for the mixed forms (float): [axis][arr][term hi/mid/lo][chunk][lane]
-constexpr int kOpsF32PerAxis = 3 * 3 * 4 * kLanes;
+// Mixed-form operand table, 32-bit words, generated on the GPU from the FP64 DFT matrices:
+//   BF16 B operands [axis][arr: R = Fr, I = Fi, NI = -Fi][pairing g][word 0..3][lane]
+//     element e of lane (j + 16q): row 4q + (e & 3), column j, term (e < 4 ? first : second) of
+//     g = 0: (Fh | Fh), g = 1: (Fm | Fm), g = 2: (Fl | Fh), g = 3: (Fm | Fl)
+//     where F = Fh + Fm + Fl is the 3-term BF16 split.
+//   FP32 B operands [axis][arr][chunk c][lane] = F[4(lane/16) + c][lane % 16]
+constexpr int kPairings = 4;
+constexpr int kBf16WordsPerAxis = 3 * kPairings * 4 * kLanes;  // 3072
+constexpr int kF32PerAxis = 3 * 4 * kLanes;            // 768
+constexpr int kOpsMixedLen = 3 * kBf16WordsPerAxis + 3 * kF32PerAxis;
+constexpr int kOpsF32Offset = 3 * kBf16WordsPerAxis;
 
 inline int64_t spectrum_len(int kind) {  // complex values per spectrum (one depth)
   return kind == KIND_3D16 ? 16 * 16 * 16 : 16 * 16;
@@ -31,7 +40,7 @@ struct TileArgs {
   double* dst;
   const double* ops;  // DFT operands (doubles)
   const double* H;    // spectrum^t / (P0 P1 [P2]) for this block, family layout, [re plane | im plane]
-  const float* opsf;  // split float operands (mixed forms)
+  const uint32_t* opsm;  // mixed-form operand table (kOpsMixedLen words)
   int64_t n0, n1, n2;       // grid sides (unused axes = 1); 1D uses n1
   int64_t tiles0, tiles1, tiles2;
   int64_t pairs;            // tile pairs in the block
@@ -39,12 +48,15 @@ struct TileArgs {
   int32_t h0, h1, h2;       // window start offset (valid region starts at h in the window)
   int32_t precision;
   int32_t pad_;
+  double dc_gain;  // K(0)^t: gain of the constant mode (mixed forms add mean * gain back in FP64)
 };
 
 // Launch helpers implemented in ffs_kernels.hip.
-hipError_t gen_operands(double* ops, float* opsf, const int32_t* axis_len, hipStream_t s);
-hipError_t gen_spectrum(double* H, int kind, const int32_t* axis_len, const double* d_taps, int32_t ntaps_per_axis,
-                        int32_t dims, int32_t convention, int32_t radius, int64_t t, hipStream_t s);
+hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hipStream_t s);
+// layout 0: FP64 accumulator order (row freq q + 4v); layout 1: FP32 accumulator order (row freq 4q + v)
+hipError_t gen_spectrum(double* H, int kind, int layout, const int32_t* axis_len, const double* d_taps,
+                        int32_t ntaps_per_axis, int32_t dims, int32_t convention, int32_t radius, int64_t t,
+                        hipStream_t s);
 hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus);
 
 }  // namespace ffsk

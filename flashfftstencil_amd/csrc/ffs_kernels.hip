@@ -22,6 +22,8 @@
 // The inverse uses conj(F) with the same operand tables and flipped combination signs.
 #include "ffs_internal.h"
 
+#include <type_traits>
+
 namespace ffsk {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -269,6 +271,298 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Mixed-precision forms, 2D 16x16 and 1D prime-factor tiles.
+//
+// FP32 accumulator layout (v_mfma_f32_16x16x32_bf16 and v_mfma_f32_16x16x4f32, measured on gfx950):
+//   D (4 regs): lane j + 16q, reg v holds D[4q + v][j]
+//   bf16 A:     lane i + 16q, element e (0..7) holds A[i][8q + e]; B: lane j + 16q, element e holds B[8q + e][j]
+//   f32 A/B:    as the FP64 form (lane i + 16k holds A[i][k], lane j + 16k holds B[k][j])
+// The K = 32 contraction of the BF16 instruction carries 16 rows x 2 split terms: element e of lane
+// group q is (row 4q + (e & 3), term e >> 2), so D's four registers feed the next stage's A operand
+// (as D^T) without any exchange, exactly like the FP64 chain.  The data (FP32) is split into 3 BF16
+// terms a = ah + am + al and the DFT matrix (FP64) into 3 terms F = Fh + Fm + Fl; each real 16x16x16
+// GEMM keeps the six products of order <= 2^-16 in three MFMAs (two products per K = 32):
+//   [ah | am] x [Fh ; Fh]  +  [ah | am] x [Fm ; Fm]  +  [ah | al] x [Fl ; Fh]
+// accumulated in FP32 (~FP32 accuracy).  NTERMS = 2 keeps a 2-term data split (16 bits) with two MFMAs,
+//   [ah | am] x [Fh ; Fh]  +  [ah | ah] x [Fm ; Fl],
+// whose error (~4e-6 of the fluctuation per block) only meets 1e-6 once the field has smoothed.
+// The FP32 form uses one v_mfma_f32_16x16x4f32 per 4-row chunk, with
+// the DFT tables stored in the permuted chunk order (rows 4k + c) that the FP32 accumulator implies.
+//
+// Every tile window is taken relative to a DC offset c (FP64, one sample of the window): the FFT path
+// runs on x - c and the store adds c * K(0)^t back in FP64 (a constant field is an eigenvector with
+// eigenvalue K(0)).  This keeps the split/rounding errors relative to the fluctuation, not to the
+// (large, slowly varying) level of the field, and is what holds the mixed forms below 1e-6 over
+// hundreds of steps.  The grid state between blocks stays FP64 in HBM.  The complex products are the plain 4-GEMM
+// form (no Gauss trick): these forms are VALU-bound, not MFMA-bound, and the 4-GEMM form needs fewer
+// operand splits and no combination adds.
+// ---------------------------------------------------------------------------------------------
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+
+#define FFS_MFMA_BF16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16((a), (b), (c), 0, 0, 0)
+#define FFS_MFMA_F32(a, b, c) __builtin_amdgcn_mfma_f32_16x16x4f32((a), (b), (c), 0, 0, 0)
+
+enum { PREC_BF16 = 1, PREC_F32 = 2, PREC_BF16X2 = 3 };
+
+template <int NTERMS>
+struct OpsBF16 {
+  bf8 r[NTERMS == 3 ? 3 : 2], i[NTERMS == 3 ? 3 : 2], ni[NTERMS == 3 ? 3 : 2];  // pairings used by the scheme
+};
+struct OpsF32 {
+  float r[4], i[4], ni[4];  // [chunk]
+};
+
+__device__ __forceinline__ bf8 as_bf8(u4 w) { return __builtin_bit_cast(bf8, w); }
+
+template <int NTERMS>
+__device__ __forceinline__ void load_ops_mixed(OpsBF16<NTERMS>& o, const uint32_t* __restrict__ t, int axis,
+                                               int lane) {
+  const uint32_t* base = t + axis * kBf16WordsPerAxis;
+  bf8* dst[3] = {o.r, o.i, o.ni};
+  constexpr int ng = NTERMS == 3 ? 3 : 2;
+#pragma unroll
+  for (int arr = 0; arr < 3; arr++)
+#pragma unroll
+    for (int g = 0; g < ng; g++) {
+      const int pg = NTERMS == 3 ? g : (g == 0 ? 0 : 3);
+      u4 w;
+#pragma unroll
+      for (int k = 0; k < 4; k++) w[k] = base[((arr * kPairings + pg) * 4 + k) * kLanes + lane];
+      dst[arr][g] = as_bf8(w);
+    }
+}
+
+__device__ __forceinline__ void load_ops_mixed(OpsF32& o, const uint32_t* __restrict__ t, int axis, int lane) {
+  const float* base = reinterpret_cast<const float*>(t + kOpsF32Offset) + axis * kF32PerAxis;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    o.r[c] = base[(0 * 4 + c) * kLanes + lane];
+    o.i[c] = base[(1 * 4 + c) * kLanes + lane];
+    o.ni[c] = base[(2 * 4 + c) * kLanes + lane];
+  }
+}
+
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+
+// Two FP32 values -> one word of two BF16 (round to nearest even, one v_cvt_pk_bf16_f32); element 0
+// (`a`) in the low half.
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{a, b}, b2));
+}
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+
+// Split of four FP32 values (rows 4q..4q+3 of this lane's column) into the A operands of the scheme.
+//   NTERMS = 3: A[0] = A[1] = [am | ah] (pairs with (Fh|Fh) and (Fm|Fm)), A[2] = [ah | al] (pairs with (Fl|Fh))
+//   NTERMS = 2: A[0] = [am | ah] (with (Fh|Fh)), A[1] = [ah | ah] (with (Fm|Fl))
+// Putting am before ah lets [am | ah] and [ah | al] share the ah registers.
+template <int NTERMS>
+__device__ __forceinline__ void split_a(const f4& x, bf8 (&A)[NTERMS]) {
+  const uint32_t h01 = pk_bf16(x[0], x[1]), h23 = pk_bf16(x[2], x[3]);
+  const float r0 = x[0] - bf_lo(h01), r1 = x[1] - bf_hi(h01), r2 = x[2] - bf_lo(h23), r3 = x[3] - bf_hi(h23);
+  const uint32_t m01 = pk_bf16(r0, r1), m23 = pk_bf16(r2, r3);
+  A[0] = as_bf8(u4{m01, m23, h01, h23});
+  if (NTERMS == 3) {
+    const uint32_t l01 = pk_bf16(r0 - bf_lo(m01), r1 - bf_hi(m01)), l23 = pk_bf16(r2 - bf_lo(m23), r3 - bf_hi(m23));
+    A[1] = A[0];
+    A[2] = as_bf8(u4{h01, h23, l01, l23});
+  } else {
+    A[1] = as_bf8(u4{h01, h23, h01, h23});
+  }
+}
+
+// out = A^T F (forward) or A^T conj(F) (INV); A complex, given as this lane's 4 rows of A's column.
+//   forward: Re = ar Fr + ai (-Fi), Im = ar Fi + ai Fr;  inverse: Re = ar Fr + ai Fi, Im = ar (-Fi) + ai Fr
+template <bool INV, int NTERMS>
+__device__ __forceinline__ void mstage(const f4& ar, const f4& ai, const OpsBF16<NTERMS>& f, f4& outr, f4& outi) {
+  bf8 R[NTERMS], I[NTERMS];
+  split_a<NTERMS>(ar, R);
+  split_a<NTERMS>(ai, I);
+  f4 re = {0.f, 0.f, 0.f, 0.f}, im = re;
+#pragma unroll
+  for (int g = 0; g < NTERMS; g++) {
+    re = FFS_MFMA_BF16(R[g], f.r[g], re);
+    im = FFS_MFMA_BF16(R[g], INV ? f.ni[g] : f.i[g], im);
+  }
+#pragma unroll
+  for (int g = 0; g < NTERMS; g++) {
+    re = FFS_MFMA_BF16(I[g], INV ? f.i[g] : f.ni[g], re);
+    im = FFS_MFMA_BF16(I[g], f.r[g], im);
+  }
+  outr = re;
+  outi = im;
+}
+
+template <bool INV, int NTERMS>
+__device__ __forceinline__ void mstage(const f4& ar, const f4& ai, const OpsF32& f, f4& outr, f4& outi) {
+  f4 re = {0.f, 0.f, 0.f, 0.f}, im = re;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    re = FFS_MFMA_F32(ar[c], f.r[c], re);
+    im = FFS_MFMA_F32(ar[c], INV ? f.ni[c] : f.i[c], im);
+    re = FFS_MFMA_F32(ai[c], INV ? f.i[c] : f.ni[c], re);
+    im = FFS_MFMA_F32(ai[c], f.r[c], im);
+  }
+  outr = re;
+  outi = im;
+}
+
+// Load the pair's windows in the FP32-accumulator order: lane (j + 16q), element r = window[4q + r][j].
+template <int KIND>
+__device__ __forceinline__ void gather_pair_m(const TileArgs& a, const PairCoord& pc, int q, int j, double (&xr)[4],
+                                              double (&xi)[4]) {
+  const double* __restrict__ src = a.src;
+  if (KIND == KIND_2D16) {
+    const int64_t rw = pc.r0 - a.h0, cwA = pc.orgA - a.h1, cwB = pc.orgB - a.h1;
+    const bool interior = rw >= 0 && rw + 16 <= a.n0 && cwA >= 0 && (pc.hasB ? cwB : cwA) + 16 <= a.n1;
+    if (interior) {
+      const double* p = src + (rw + 4 * q) * a.n1 + cwA + j;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        xr[r] = p[r * a.n1];
+        xi[r] = pc.hasB ? p[r * a.n1 + a.V1] : 0.0;
+      }
+    } else {
+      const int64_t cA = wrap_near(cwA + j, a.n1);
+      const int64_t cB = wrap_near(cwB + j, a.n1);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t row = wrap_near(rw + 4 * q + r, a.n0) * a.n1;
+        xr[r] = src[row + cA];
+        xi[r] = pc.hasB ? src[row + cB] : 0.0;
+      }
+    }
+  } else {
+    const int64_t wA = pc.orgA - a.h1, wB = pc.orgB - a.h1;
+    const bool interior = wA >= 0 && (pc.hasB ? wB : wA) + 240 <= a.n1;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = 4 * q + r;
+      const int n = (225 * j + 16 * row) % 240;
+      const bool pad = row >= 15;
+      if (interior) {
+        xr[r] = pad ? 0.0 : src[wA + n];
+        xi[r] = (pad || !pc.hasB) ? 0.0 : src[wB + n];
+      } else {
+        xr[r] = pad ? 0.0 : src[wrap_near(wA + n, a.n1)];
+        xi[r] = (pad || !pc.hasB) ? 0.0 : src[wrap_near(wB + n, a.n1)];
+      }
+    }
+  }
+}
+
+template <int KIND>
+__device__ __forceinline__ void store_pair_m(const TileArgs& a, const PairCoord& pc, int q, int j, const f4& yr,
+                                             const f4& yi, double addA, double addB) {
+  if (KIND == KIND_2D16) {
+    const int64_t cA = pc.orgA + j - a.h1, cB = pc.orgB + j - a.h1;
+    const bool colok = j >= a.h1 && j < a.h1 + a.V1;
+    const bool okA = colok && cA < a.n1;
+    const bool okB = colok && pc.hasB && cB < a.n1;
+    double* base = a.dst + (pc.r0 + 4 * q - a.h0) * a.n1;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = 4 * q + v;
+      if (row >= a.h0 && row < a.h0 + a.V0 && pc.r0 + row - a.h0 < a.n0) {
+        double* drow = base + (int64_t)v * a.n1;
+        if (okA) drow[cA] = (double)yr[v] + addA;
+        if (okB) drow[cB] = (double)yi[v] + addB;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = 4 * q + v;
+      if (row < 15) {
+        const int n = (225 * j + 16 * row) % 240;
+        if (n >= a.h1 && n < a.h1 + a.V1) {
+          const int64_t pA = pc.orgA + n - a.h1, pB = pc.orgB + n - a.h1;
+          if (pA < a.n1) a.dst[pA] = (double)yr[v] + addA;
+          if (pc.hasB && pB < a.n1) a.dst[pB] = (double)yi[v] + addB;
+        }
+      }
+    }
+  }
+}
+
+// Wave-uniform copy of lane 0's value.
+__device__ __forceinline__ double lane0(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b), hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// One tile pair through the mixed pipeline.  The DC offset of each window is its first sample (row 0,
+// column 0: lane 0, element 0), which tracks the local mean once the field is smooth and bounds the
+// fluctuation by the window's range before.
+template <int KIND, int NTERMS, typename Ops>
+__device__ __forceinline__ void mixed_pair(const TileArgs& a, const PairCoord& pc, int q, int j, const double (&xr)[4],
+                                           const double (&xi)[4], const Ops& f0, const Ops& g1, const f4& hr,
+                                           const f4& hi) {
+  const double cA = lane0(xr[0]), cB = lane0(xi[0]);
+  f4 ar, ai;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    ar[r] = (float)(xr[r] - cA);
+    ai[r] = (float)(xi[r] - cB);
+  }
+  f4 sr, si;
+  mstage<false, NTERMS>(ar, ai, f0, sr, si);
+  mstage<false, NTERMS>(sr, si, g1, ar, ai);  // Y[k0 = 4q+v][k1 = j]
+  sr = ar * hr - ai * hi;
+  si = ar * hi + ai * hr;
+  mstage<true, NTERMS>(sr, si, f0, ar, ai);
+  mstage<true, NTERMS>(ar, ai, g1, sr, si);  // X'[n0 = 4q+v][n1 = j]
+  store_pair_m<KIND>(a, pc, q, j, sr, si, cA * a.dc_gain, cB * a.dc_gain);
+}
+
+template <int KIND, int PREC, int NTERMS>
+__global__ __launch_bounds__(256) void k_tile16_mixed(TileArgs a) {
+  typedef typename std::conditional<PREC == PREC_BF16, OpsBF16<NTERMS>, OpsF32>::type Ops;
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, j = lane & 15;
+  const int64_t W = (int64_t)gridDim.x * 4;
+  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+  Ops f0, f1;
+  load_ops_mixed(f0, a.opsm, 0, lane);
+  if (KIND != KIND_2D16) load_ops_mixed(f1, a.opsm, 1, lane);
+  const Ops& g1 = (KIND == KIND_2D16) ? f0 : f1;
+  f4 hr, hi;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    hr[v] = (float)a.H[v * kLanes + lane];
+    hi[v] = (float)a.H[256 + v * kLanes + lane];
+  }
+
+  // Two register sets alternate (unrolled by 2): the next pair's window loads while this one computes.
+  int64_t tp = gw;
+  if (tp >= a.pairs) return;
+  PairCoord pa = pair_coord<KIND>(a, tp), pb = pa;
+  double ar[4], ai[4], br[4] = {0.0, 0.0, 0.0, 0.0}, bi[4] = {0.0, 0.0, 0.0, 0.0};
+  gather_pair_m<KIND>(a, pa, q, j, ar, ai);
+  while (true) {
+    if (tp + W < a.pairs) {
+      pb = pair_coord<KIND>(a, tp + W);
+      gather_pair_m<KIND>(a, pb, q, j, br, bi);
+    }
+    mixed_pair<KIND, NTERMS>(a, pa, q, j, ar, ai, f0, g1, hr, hi);
+    tp += W;
+    if (tp >= a.pairs) break;
+    if (tp + W < a.pairs) {
+      pa = pair_coord<KIND>(a, tp + W);
+      gather_pair_m<KIND>(a, pa, q, j, ar, ai);
+    }
+    mixed_pair<KIND, NTERMS>(a, pb, q, j, br, bi, f0, g1, hr, hi);
+    tp += W;
+    if (tp >= a.pairs) break;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Plan-time generation (FP64 on the GPU).
 // ---------------------------------------------------------------------------------------------
 __global__ void k_gen_operands(double* ops, int32_t L0, int32_t L1, int32_t L2) {
@@ -293,11 +587,69 @@ __global__ void k_gen_operands(double* ops, int32_t L0, int32_t L1, int32_t L2) 
   base[(2 * 4 + c) * kLanes + lane] = fi - fr;
 }
 
-// K(k) = sum_taps w[a] exp(-2 pi i sum_ax (d_ax(a) k_ax mod L_ax)/L_ax),  d = a (ref) or r - a (centred);
+// 3-term BF16 split of an FP64 value: x ~ hi + mid + lo, each rounded to nearest even.
+__device__ __forceinline__ void split3_bf16(double x, __bf16& h, __bf16& m, __bf16& l) {
+  h = (__bf16)(float)x;
+  const double r1 = x - (double)(float)h;
+  m = (__bf16)(float)r1;
+  const double r2 = r1 - (double)(float)m;
+  l = (__bf16)(float)r2;
+}
+
+__device__ __forceinline__ void dft_entry(int row, int col, int L, double& fr, double& fi) {
+  fr = 0.0;
+  fi = 0.0;
+  if (row < L && col < L) {
+    const int m = (row * col) % L;
+    double s, c;
+    sincospi(-2.0 * (double)m / (double)L, &s, &c);
+    fr = c;
+    fi = s;
+  }
+}
+
+// Mixed-form operand tables (layout in ffs_internal.h); one thread per (axis, arr, lane).
+__global__ void k_gen_operands_mixed(uint32_t* t, int32_t L0, int32_t L1, int32_t L2) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 3 * 3 * kLanes) return;
+  const int axis = idx / (3 * kLanes), arr = (idx / kLanes) % 3, lane = idx % kLanes;
+  const int L = axis == 0 ? L0 : (axis == 1 ? L1 : L2);
+  const int q = lane >> 4, j = lane & 15;
+  // BF16 pairings: (h|h), (m|m), (l|h), (m|l)
+  for (int g = 0; g < kPairings; g++) {
+    for (int w = 0; w < 4; w++) {
+      uint32_t word = 0;
+      for (int half = 0; half < 2; half++) {
+        const int e = 2 * w + half;
+        double fr, fi;
+        dft_entry(4 * q + (e & 3), j, L, fr, fi);
+        const double v = arr == 0 ? fr : (arr == 1 ? fi : -fi);
+        __bf16 h, m, l;
+        split3_bf16(v, h, m, l);
+        const __bf16 first[kPairings] = {h, m, l, m}, second[kPairings] = {h, m, h, l};
+        const __bf16 pick = e < 4 ? first[g] : second[g];
+        const uint32_t bits = (uint32_t)__builtin_bit_cast(uint16_t, pick);
+        word |= bits << (16 * half);
+      }
+      t[axis * kBf16WordsPerAxis + ((arr * kPairings + g) * 4 + w) * kLanes + lane] = word;
+    }
+  }
+  // FP32 chunks: chunk c of lane (j + 16q) = F[4q + c][j]
+  float* tf = reinterpret_cast<float*>(t + kOpsF32Offset) + axis * kF32PerAxis;
+  for (int c = 0; c < 4; c++) {
+    double fr, fi;
+    dft_entry(4 * q + c, j, L, fr, fi);
+    const double v = arr == 0 ? fr : (arr == 1 ? fi : -fi);
+    tf[(arr * 4 + c) * kLanes + lane] = (float)v;
+  }
+}
+
+// K(k) = sum_taps w[a]// K(k) = sum_taps w[a] exp(-2 pi i sum_ax (d_ax(a) k_ax mod L_ax)/L_ax),  d = a (ref) or r - a (centred);
 // H = K^t / prod(L) by repeated squaring in complex FP64.  `kind` fixes which frequency sits at
 // entry e of the family layout.
-__global__ void k_gen_spectrum(double* H, int kind, int32_t L0, int32_t L1, int32_t L2, const double* taps,
-                               int32_t K, int32_t dims, int32_t convention, int32_t radius, int64_t t) {
+__global__ void k_gen_spectrum(double* H, int kind, int layout, int32_t L0, int32_t L1, int32_t L2,
+                               const double* taps, int32_t K, int32_t dims, int32_t convention, int32_t radius,
+                               int64_t t) {
   const int64_t len = kind == KIND_3D16 ? 4096 : 256;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= len) return;
@@ -310,7 +662,7 @@ __global__ void k_gen_spectrum(double* H, int kind, int32_t L0, int32_t L1, int3
   } else {
     v = (int)(e / kLanes);
   }
-  const int krow = q + 4 * v;  // frequency on the row axis of the family layout
+  const int krow = layout == 0 ? q + 4 * v : 4 * q + v;  // frequency on the row axis of the family layout
   double kr = 0.0, ki = 0.0;
   bool zero = false;
   int kax[3] = {0, 0, 0};
@@ -379,16 +731,18 @@ __global__ void k_gen_spectrum(double* H, int kind, int32_t L0, int32_t L1, int3
   H[len + e] = zero ? 0.0 : ki;
 }
 
-hipError_t gen_operands(double* ops, float* /*opsf*/, const int32_t* axis_len, hipStream_t s) {
+hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hipStream_t s) {
   hipLaunchKernelGGL(k_gen_operands, dim3(3), dim3(256), 0, s, ops, axis_len[0], axis_len[1], axis_len[2]);
+  if (opsm)
+    hipLaunchKernelGGL(k_gen_operands_mixed, dim3(3), dim3(192), 0, s, opsm, axis_len[0], axis_len[1], axis_len[2]);
   return hipGetLastError();
 }
 
-hipError_t gen_spectrum(double* H, int kind, const int32_t* axis_len, const double* d_taps, int32_t K, int32_t dims,
-                        int32_t convention, int32_t radius, int64_t t, hipStream_t s) {
+hipError_t gen_spectrum(double* H, int kind, int layout, const int32_t* axis_len, const double* d_taps, int32_t K,
+                        int32_t dims, int32_t convention, int32_t radius, int64_t t, hipStream_t s) {
   const int len = kind == KIND_3D16 ? 4096 : 256;
-  hipLaunchKernelGGL(k_gen_spectrum, dim3((len + 255) / 256), dim3(256), 0, s, H, kind, axis_len[0], axis_len[1],
-                     axis_len[2], d_taps, K, dims, convention, radius, t);
+  hipLaunchKernelGGL(k_gen_spectrum, dim3((len + 255) / 256), dim3(256), 0, s, H, kind, layout, axis_len[0],
+                     axis_len[1], axis_len[2], d_taps, K, dims, convention, radius, t);
   return hipGetLastError();
 }
 
@@ -398,10 +752,25 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
   int64_t blocks = (waves_needed + 3) / 4;
   const int64_t cap = (int64_t)num_cus * 8;  // 8 workgroups (32 waves) per CU at most, grid-stride beyond
   if (blocks > cap) blocks = cap;
+  const dim3 g((unsigned)blocks), b(256);
   if (kind == KIND_2D16) {
-    hipLaunchKernelGGL(k_tile16_f64<KIND_2D16>, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    if (a.precision == PREC_BF16)
+      hipLaunchKernelGGL((k_tile16_mixed<KIND_2D16, PREC_BF16, 3>), g, b, 0, s, a);
+    else if (a.precision == PREC_BF16X2)
+      hipLaunchKernelGGL((k_tile16_mixed<KIND_2D16, PREC_BF16, 2>), g, b, 0, s, a);
+    else if (a.precision == PREC_F32)
+      hipLaunchKernelGGL((k_tile16_mixed<KIND_2D16, PREC_F32, 1>), g, b, 0, s, a);
+    else
+      hipLaunchKernelGGL(k_tile16_f64<KIND_2D16>, g, b, 0, s, a);
   } else if (kind == KIND_1DPFA) {
-    hipLaunchKernelGGL(k_tile16_f64<KIND_1DPFA>, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    if (a.precision == PREC_BF16)
+      hipLaunchKernelGGL((k_tile16_mixed<KIND_1DPFA, PREC_BF16, 3>), g, b, 0, s, a);
+    else if (a.precision == PREC_BF16X2)
+      hipLaunchKernelGGL((k_tile16_mixed<KIND_1DPFA, PREC_BF16, 2>), g, b, 0, s, a);
+    else if (a.precision == PREC_F32)
+      hipLaunchKernelGGL((k_tile16_mixed<KIND_1DPFA, PREC_F32, 1>), g, b, 0, s, a);
+    else
+      hipLaunchKernelGGL(k_tile16_f64<KIND_1DPFA>, g, b, 0, s, a);
   } else {
     return hipErrorInvalidValue;
   }

@@ -1,0 +1,99 @@
+"""GPU parity of the mixed-precision forms against the FP64 direct-sweep oracle, through the C ABI.
+
+Bar (stated for the mixed forms): max|gpu - oracle| / max|oracle| <= 1e-6, on the reference's shapes
+and boundary convention, including long horizons (T = 256) where per-block rounding accumulates.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_MIXED = 1e-6
+
+
+def _run(ffs, torch, x, radius, w, steps, **kw):
+    plan = ffs.Plan(x.ndim, x.shape, w, radius, steps, **kw)
+    d_in = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    d_out = torch.empty_like(d_in)
+    plan.run(d_in, d_out)
+    torch.cuda.synchronize()
+    info = plan.info
+    plan.close()
+    return d_out.cpu().numpy(), info
+
+
+@pytest.mark.parametrize("form", ["bf16", "f32"])
+@pytest.mark.parametrize("shape", ["Heat-2D", "Box2D9P", "Star2D9P", "Box2D25P"])
+@pytest.mark.parametrize("size", [(37, 53), (16, 16), (64, 40), (3, 5)])
+@pytest.mark.parametrize("steps", [1, 3, 8])
+def test_2d_mixed_named(ffs, oracle, torch_cuda, form, shape, size, steps):
+    _, r, w, _ = ffs.named_stencil(shape, seed=21)
+    x = oracle.uniform(4321 + size[0] * 7 + size[1], size)
+    got, info = _run(ffs, torch_cuda, x, r, w, steps, precision=form)
+    ref = oracle.sweep(x, r, w, steps)
+    _, rel, _ = oracle.errors(got, ref)
+    assert rel <= TOL_MIXED, (form, shape, size, steps, info, rel)
+
+
+@pytest.mark.parametrize("form", ["bf16", "f32", "bf16x2"])
+def test_2d_mixed_long_horizon(ffs, oracle, torch_cuda, form):
+    _, r, w, _ = ffs.named_stencil("Box2D25P", seed=0x5EED0002)
+    x = oracle.uniform(0x5EED0003, (256, 256))
+    got, info = _run(ffs, torch_cuda, x, r, w, 256, precision=form)
+    ref = oracle.sweep(x, r, w, 256)
+    _, rel, l2 = oracle.errors(got, ref)
+    print(form, info["fuse"], rel, l2)
+    assert rel <= TOL_MIXED, (form, info, rel, l2)
+
+
+@pytest.mark.parametrize("form", ["bf16", "f32"])
+@pytest.mark.parametrize("convention", ["ref", "centred"])
+@pytest.mark.parametrize("fuse", [1, 2, 3])
+def test_2d_mixed_fuse_convention(ffs, oracle, torch_cuda, form, convention, fuse):
+    rng = np.random.default_rng(40 + fuse)
+    w = rng.random(9)
+    w /= w.sum()
+    x = oracle.uniform(77, (50, 70))
+    got, info = _run(ffs, torch_cuda, x, 1, w, 2 * fuse + 1, precision=form, convention=convention, fuse=fuse)
+    ref = oracle.sweep(x, 1, w, 2 * fuse + 1, convention=0 if convention == "ref" else 1)
+    assert oracle.errors(got, ref)[1] <= TOL_MIXED
+
+
+@pytest.mark.parametrize("form", ["bf16", "f32"])
+def test_2d_mixed_gain_not_one(ffs, oracle, torch_cuda, form):
+    # weights summing to 1.3: the constant mode is carried in FP64 with gain (sum w)^t
+    rng = np.random.default_rng(9)
+    w = rng.random(25)
+    w *= 1.3 / w.sum()
+    x = oracle.uniform(8, (60, 45)) + 5.0
+    got, _ = _run(ffs, torch_cuda, x, 2, w, 4, precision=form)
+    ref = oracle.sweep(x, 2, w, 4)
+    assert oracle.errors(got, ref)[1] <= TOL_MIXED
+
+
+@pytest.mark.parametrize("form", ["bf16", "f32"])
+@pytest.mark.parametrize("shape", ["Heat-1D", "1D5P", "1D7P"])
+@pytest.mark.parametrize("n", [5, 240, 1001, 9999])
+def test_1d_mixed_named(ffs, oracle, torch_cuda, form, shape, n):
+    _, r, w, _ = ffs.named_stencil(shape, seed=5)
+    x = oracle.uniform(1000 + n, (n,))
+    for steps in (1, 64):
+        got, info = _run(ffs, torch_cuda, x, r, w, steps, precision=form)
+        ref = oracle.sweep(x, r, w, steps)
+        _, rel, _ = oracle.errors(got, ref)
+        assert rel <= TOL_MIXED, (form, shape, n, steps, info, rel)
+
+
+def test_mixed_deterministic(ffs, oracle, torch_cuda):
+    torch = torch_cuda
+    _, r, w, _ = ffs.named_stencil("Box2D9P", seed=2)
+    x = oracle.uniform(6, (300, 200))
+    d_in = torch.from_numpy(x).cuda()
+    plan = ffs.Plan(2, x.shape, w, r, 20, precision="bf16")
+    a = torch.empty_like(d_in)
+    b = torch.empty_like(d_in)
+    plan.run(d_in, a)
+    plan.run(d_in, b)
+    torch.cuda.synchronize()
+    plan.close()
+    assert torch.equal(a, b)
