@@ -129,13 +129,19 @@ ffs_status choose_config(ffs_plan* pl) {
     pl->L[0] = 15;
     pl->L[1] = 16;
   } else if (p.dims == 2) {
-    if (p.tile != 0 && p.tile != 16) return fail(FFS_ERR_UNSUPPORTED, "2D tile must be 16, got %d", p.tile);
-    pl->kind = ffsk::KIND_2D16;
-    pl->P[0] = pl->P[1] = 16;
+    const bool bf = p.precision == FFS_PREC_MIXED_BF16 || p.precision == FFS_PREC_MIXED_BF16X2;
+    const int P = p.tile != 0 ? p.tile : (bf ? 32 : 16);
+    if (!(P == 16 || (P == 32 && bf)))
+      return fail(FFS_ERR_UNSUPPORTED, "2D tile %d not available for this precision (16; 32 with the BF16 forms)", P);
+    pl->kind = P == 32 ? ffsk::KIND_2D32 : ffsk::KIND_2D16;
+    pl->P[0] = pl->P[1] = P;
+    pl->L[0] = pl->L[1] = P;
   } else {
-    return fail(FFS_ERR_UNSUPPORTED, "3D is not built yet");
+    if (p.tile != 0 && p.tile != 16) return fail(FFS_ERR_UNSUPPORTED, "3D tile must be 16, got %d", p.tile);
+    pl->kind = ffsk::KIND_3D16;
+    pl->P[0] = pl->P[1] = pl->P[2] = 16;
   }
-  const int Pmain = p.dims == 1 ? 240 : 16;
+  const int Pmain = p.dims == 1 ? 240 : pl->P[0];
   const int vmin = p.dims == 1 ? 16 : 2;
   // largest admissible depth: V = P - 2 r t >= vmin
   int64_t tmax = r == 0 ? (int64_t)1 << 40 : (Pmain - vmin) / (2 * r);
@@ -152,7 +158,8 @@ ffs_status choose_config(ffs_plan* pl) {
     // MI355X) and the HBM cost is 16 B per valid point at ~3.26 B/cycle/SIMD (8 TB/s over 1024 SIMDs
     // at 2.4 GHz).
     const int dd = p.dims == 1 ? 1 : p.dims;
-    const double compute = p.precision == FFS_PREC_FP64 ? 4700.0 : (p.precision == FFS_PREC_MIXED_F32 ? 2600.0 : 1100.0);
+    double compute = p.precision == FFS_PREC_FP64 ? 4700.0 : (p.precision == FFS_PREC_MIXED_F32 ? 2600.0 : 2500.0);
+    if (pl->kind == ffsk::KIND_2D32) compute = p.precision == FFS_PREC_MIXED_BF16 ? 12000.0 : 9000.0;
     const double hbm_per_point = 2.0 * 16.0 / 3.26;
     double best = -1;
     tbest = 1;
@@ -193,12 +200,13 @@ ffs_status choose_config(ffs_plan* pl) {
     pl->pairs = (pl->tiles[1] + 1) / 2;
   } else {
     for (int d = 0; d < p.dims; d++) {
-      pl->V[d] = (int32_t)(16 - 2 * r * pl->t);
+      pl->V[d] = (int32_t)(Pmain - 2 * r * pl->t);
       pl->h_main[d] = (int32_t)(hi * pl->t);
       pl->h_tail[d] = (int32_t)(hi * pl->t_tail);
       pl->tiles[d] = (p.sizes[d] + pl->V[d] - 1) / pl->V[d];
     }
     if (p.dims == 2) pl->pairs = pl->tiles[0] * ((pl->tiles[1] + 1) / 2);
+    if (p.dims == 3) pl->pairs = pl->tiles[0] * pl->tiles[1] * ((pl->tiles[2] + 1) / 2);
   }
   return FFS_OK;
 }
@@ -396,7 +404,8 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
     if (x != hipSuccess && e == hipSuccess) e = x;
   };
   chk(hipMalloc(&pl->d_ops, sizeof(double) * ffsk::kOpsLen));
-  if (p->precision != FFS_PREC_FP64) chk(hipMalloc(&pl->d_opsm, sizeof(uint32_t) * ffsk::kOpsMixedLen));
+  if (p->precision != FFS_PREC_FP64)
+    chk(hipMalloc(&pl->d_opsm, sizeof(uint32_t) * std::max(ffsk::kOpsMixedLen, ffsk::kOps32Words)));
   chk(hipMalloc(&pl->d_H, sizeof(double) * 4 * pl->hlen));
   chk(hipMalloc(&pl->d_taps, sizeof(double) * ntaps));
   if (pl->nblocks >= 2) chk(hipMalloc(&pl->d_scratch, sizeof(double) * npoints));
@@ -407,7 +416,10 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
   }
   chk(hipMemcpy(pl->d_taps, pl->weights.data(), sizeof(double) * ntaps, hipMemcpyHostToDevice));
   const int layout = p->precision == FFS_PREC_FP64 ? 0 : 1;
-  chk(ffsk::gen_operands(pl->d_ops, pl->d_opsm, pl->L, nullptr));
+  if (pl->kind == ffsk::KIND_2D32)
+    chk(ffsk::gen_operands32(pl->d_opsm, nullptr));
+  else
+    chk(ffsk::gen_operands(pl->d_ops, pl->d_opsm, pl->L, nullptr));
   chk(ffsk::gen_spectrum(pl->d_H, pl->kind, layout, pl->L, pl->d_taps, K, p->dims, p->convention, p->radius, pl->t,
                          nullptr));
   chk(ffsk::gen_spectrum(pl->d_H + 2 * pl->hlen, pl->kind, layout, pl->L, pl->d_taps, K, p->dims, p->convention,
@@ -453,7 +465,7 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   } else {
     info->tiles = 1;
     for (int d = 0; d < 3; d++) {
-      info->tile[d] = d < p.dims ? 16 : 1;
+      info->tile[d] = d < p.dims ? pl->P[d] : 1;
       info->valid[d] = d < p.dims ? pl->V[d] : 1;
       if (d < p.dims) info->tiles *= pl->tiles[d];
     }
@@ -473,7 +485,9 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   if (p.precision == FFS_PREC_MIXED_BF16) per_cplx = 4.0 * 3.0 * 2.0 * 16 * 16 * 32;
   if (p.precision == FFS_PREC_MIXED_BF16X2) per_cplx = 4.0 * 2.0 * 2.0 * 16 * 16 * 32;
   if (p.precision == FFS_PREC_MIXED_F32) per_cplx = 4.0 * gemm;
-  const double per_pair = 2.0 * dd * per_cplx * std::pow(16.0, dd - 2);
+  double per_pair = 2.0 * dd * per_cplx * std::pow(16.0, dd - 2);
+  if (pl->kind == ffsk::KIND_2D32)  // 4 output blocks x 4 real GEMMs x (6 or 4) K=32 MFMAs, 4 stages
+    per_pair = 4.0 * 4.0 * 4.0 * (p.precision == FFS_PREC_MIXED_BF16 ? 6.0 : 4.0) * 2.0 * 16 * 16 * 32;
   info->flops_per_block = per_pair * (double)pl->pairs;
   info->bytes_per_block = 16.0 * (double)pl->npoints;
   return FFS_OK;

@@ -10,6 +10,7 @@ enum TileKind : int32_t {
   KIND_2D16 = 0,   // 2D, 16x16 window
   KIND_1DPFA = 1,  // 1D, 240-sample window as a 15x16 prime-factor (Good-Thomas) matrix
   KIND_3D16 = 2,   // 3D, 16x16x16 window (one workgroup of 4 waves per tile pair)
+  KIND_2D32 = 3,   // 2D, 32x32 window (mixed BF16 form: the contraction fills K = 32 of the MFMA)
 };
 
 // Operand-table layout in doubles (one table per plan, generated on the GPU in FP64):
@@ -32,8 +33,12 @@ constexpr int kOpsMixedLen = 3 * kBf16WordsPerAxis + 3 * kF32PerAxis;
 constexpr int kOpsF32Offset = 3 * kBf16WordsPerAxis;
 
 inline int64_t spectrum_len(int kind) {  // complex values per spectrum (one depth)
-  return kind == KIND_3D16 ? 16 * 16 * 16 : 16 * 16;
+  return kind == KIND_3D16 ? 16 * 16 * 16 : (kind == KIND_2D32 ? 32 * 32 : 16 * 16);
 }
+
+// 32-point BF16 B operands (KIND_2D32): [arr: R, I, NI][column block Kb][term h, m, l][word 0..3][lane];
+// element e of lane (j + 16q) is row 16 (e >> 2) + 4q + (e & 3), column 16 Kb + j of that term.
+constexpr int kOps32Words = 3 * 2 * 3 * 4 * kLanes;  // 4608
 
 struct TileArgs {
   const double* src;
@@ -53,6 +58,7 @@ struct TileArgs {
 
 // Launch helpers implemented in ffs_kernels.hip.
 hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hipStream_t s);
+hipError_t gen_operands32(uint32_t* ops32, hipStream_t s);
 // layout 0: FP64 accumulator order (row freq q + 4v); layout 1: FP32 accumulator order (row freq 4q + v)
 hipError_t gen_spectrum(double* H, int kind, int layout, const int32_t* axis_len, const double* d_taps,
                         int32_t ntaps_per_axis, int32_t dims, int32_t convention, int32_t radius, int64_t t,

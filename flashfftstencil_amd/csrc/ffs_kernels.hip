@@ -22,6 +22,7 @@
 // The inverse uses conj(F) with the same operand tables and flipped combination signs.
 #include "ffs_internal.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace ffsk {
@@ -563,6 +564,404 @@ __global__ __launch_bounds__(256) void k_tile16_mixed(TileArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// 2D 32x32 tiles, mixed BF16 form.
+//
+// A 32x32 matrix is held as 4 blocks (R, C) of 16x16 in FP32-accumulator order:
+//   block b = 2R + C, lane (j + 16q), reg v  <->  M[16R + 4q + v][16C + j].
+// A stage Out = Z^T F contracts all 32 rows of Z: output block (I, Kb) = sum_R Z(R, I)^T F(R, Kb).  The
+// BF16 A operand of one MFMA spans K = 32 = both row blocks: element e of lane (i + 16q) is
+// Z[16 (e >> 2) + 4q + (e & 3)][16 I + i], i.e. registers (e & 3) of blocks (0, I) and (1, I), so the
+// chain again needs no exchange between lanes.  Each real 16x16 output block of a GEMM keeps the
+// split products of order <= 2^-16: (hh, hm, mh, hl, mm, lh) for NTERMS = 3 (6 MFMAs), (hh, hm, mh, hl)
+// for NTERMS = 2 (4 MFMAs).  DFT operands (18 KB) and the spectrum (8 KB) are staged once per
+// workgroup in LDS.  With the 32-point window the fused depth can be 2-3x deeper than at 16 points
+// for the same valid fraction, which cuts the split work and the HBM traffic per stencil step.
+// ---------------------------------------------------------------------------------------------
+struct Blk4 {
+  f4 b[4];
+};
+
+template <int NTERMS>
+__device__ __forceinline__ void split8(const f4& x0, const f4& x1, bf8& h, bf8& m, bf8& l) {
+  const uint32_t h0 = pk_bf16(x0[0], x0[1]), h1 = pk_bf16(x0[2], x0[3]);
+  const uint32_t h2 = pk_bf16(x1[0], x1[1]), h3 = pk_bf16(x1[2], x1[3]);
+  const float r0 = x0[0] - bf_lo(h0), r1 = x0[1] - bf_hi(h0), r2 = x0[2] - bf_lo(h1), r3 = x0[3] - bf_hi(h1);
+  const float r4 = x1[0] - bf_lo(h2), r5 = x1[1] - bf_hi(h2), r6 = x1[2] - bf_lo(h3), r7 = x1[3] - bf_hi(h3);
+  const uint32_t m0 = pk_bf16(r0, r1), m1 = pk_bf16(r2, r3), m2 = pk_bf16(r4, r5), m3 = pk_bf16(r6, r7);
+  h = as_bf8(u4{h0, h1, h2, h3});
+  m = as_bf8(u4{m0, m1, m2, m3});
+  if (NTERMS == 3) {
+    const uint32_t l0 = pk_bf16(r0 - bf_lo(m0), r1 - bf_hi(m0)), l1 = pk_bf16(r2 - bf_lo(m1), r3 - bf_hi(m1));
+    const uint32_t l2 = pk_bf16(r4 - bf_lo(m2), r5 - bf_hi(m2)), l3 = pk_bf16(r6 - bf_lo(m3), r7 - bf_hi(m3));
+    l = as_bf8(u4{l0, l1, l2, l3});
+  }
+}
+
+// acc += (ah + am + al) x (Fh + Fm + Fl), products of order <= 2^-16 only.
+template <int NTERMS>
+__device__ __forceinline__ f4 prod32(const bf8& ah, const bf8& am, const bf8& al, const bf8 (&F)[3], f4 acc) {
+  acc = FFS_MFMA_BF16(ah, F[0], acc);
+  acc = FFS_MFMA_BF16(ah, F[1], acc);
+  acc = FFS_MFMA_BF16(am, F[0], acc);
+  acc = FFS_MFMA_BF16(ah, F[2], acc);
+  if (NTERMS == 3) {
+    acc = FFS_MFMA_BF16(am, F[1], acc);
+    acc = FFS_MFMA_BF16(al, F[0], acc);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ void load_f32ops(const u4* __restrict__ sF, int arr, int Kb, int lane, bf8 (&F)[3]) {
+#pragma unroll
+  for (int t = 0; t < 3; t++) F[t] = as_bf8(sF[((arr * 2 + Kb) * 3 + t) * kLanes + lane]);
+}
+
+template <bool INV, int NTERMS>
+__device__ __forceinline__ void stage32(const Blk4& zr, const Blk4& zi, const u4* __restrict__ sF, int lane, Blk4& outr,
+                                        Blk4& outi) {
+#pragma unroll
+  for (int I = 0; I < 2; I++) {
+    bf8 rh, rm, rl, ih, im, il;
+    split8<NTERMS>(zr.b[I], zr.b[2 + I], rh, rm, rl);
+    split8<NTERMS>(zi.b[I], zi.b[2 + I], ih, im, il);
+#pragma unroll
+    for (int Kb = 0; Kb < 2; Kb++) {
+      bf8 Fr[3], Fi[3], Fn[3];
+      load_f32ops(sF, 0, Kb, lane, Fr);
+      load_f32ops(sF, 1, Kb, lane, Fi);
+      load_f32ops(sF, 2, Kb, lane, Fn);
+      f4 re = {0.f, 0.f, 0.f, 0.f}, iv = re;
+      // forward: Re = zr Fr + zi (-Fi), Im = zr Fi + zi Fr;  inverse: Re = zr Fr + zi Fi, Im = zr (-Fi) + zi Fr
+      re = prod32<NTERMS>(rh, rm, rl, Fr, re);
+      iv = prod32<NTERMS>(rh, rm, rl, INV ? Fn : Fi, iv);
+      re = prod32<NTERMS>(ih, im, il, INV ? Fi : Fn, re);
+      iv = prod32<NTERMS>(ih, im, il, Fr, iv);
+      outr.b[2 * I + Kb] = re;
+      outi.b[2 * I + Kb] = iv;
+    }
+  }
+}
+
+// Window values in block order: x[b * 4 + r] = window[16 (b >> 1) + 4q + r][16 (b & 1) + j].
+__device__ __forceinline__ void gather32(const TileArgs& a, const PairCoord& pc, int q, int j, double (&xa)[16],
+                                         double (&xb)[16]) {
+  const double* __restrict__ src = a.src;
+  const int64_t rw = pc.r0 - a.h0, cwA = pc.orgA - a.h1, cwB = pc.orgB - a.h1;
+  const bool interior = rw >= 0 && rw + 32 <= a.n0 && cwA >= 0 && (pc.hasB ? cwB : cwA) + 32 <= a.n1;
+  if (interior) {
+    const double* p = src + (rw + 4 * q) * a.n1 + cwA + j;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t off = (int64_t)(16 * (b >> 1) + r) * a.n1 + 16 * (b & 1);
+        xa[b * 4 + r] = p[off];
+        xb[b * 4 + r] = pc.hasB ? p[off + a.V1] : 0.0;
+      }
+  } else {
+#pragma unroll
+    for (int C = 0; C < 2; C++) {
+      const int64_t cA = wrap_near(cwA + 16 * C + j, a.n1);
+      const int64_t cB = wrap_near(cwB + 16 * C + j, a.n1);
+#pragma unroll
+      for (int R = 0; R < 2; R++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int64_t row = wrap_near(rw + 16 * R + 4 * q + r, a.n0) * a.n1;
+          xa[(2 * R + C) * 4 + r] = src[row + cA];
+          xb[(2 * R + C) * 4 + r] = pc.hasB ? src[row + cB] : 0.0;
+        }
+    }
+  }
+}
+
+__device__ __forceinline__ void store32(const TileArgs& a, const PairCoord& pc, int q, int j, const Blk4& yr,
+                                        const Blk4& yi, double addA, double addB) {
+#pragma unroll
+  for (int C = 0; C < 2; C++) {
+    const int wc = 16 * C + j;
+    const int64_t cA = pc.orgA + wc - a.h1, cB = pc.orgB + wc - a.h1;
+    const bool colok = wc >= a.h1 && wc < a.h1 + a.V1;
+    const bool okA = colok && cA < a.n1;
+    const bool okB = colok && pc.hasB && cB < a.n1;
+#pragma unroll
+    for (int R = 0; R < 2; R++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = 16 * R + 4 * q + v;
+        const int64_t y = pc.r0 + row - a.h0;
+        if (row >= a.h0 && row < a.h0 + a.V0 && y < a.n0) {
+          double* drow = a.dst + y * a.n1;
+          if (okA) drow[cA] = (double)yr.b[2 * R + C][v] + addA;
+          if (okB) drow[cB] = (double)yi.b[2 * R + C][v] + addB;
+        }
+      }
+  }
+}
+
+template <int NTERMS>
+__global__ __launch_bounds__(256, 2) void k_tile32_bf16(TileArgs a) {
+  __shared__ u4 sF[3 * 2 * 3 * kLanes];  // B operands, 18 KB
+  __shared__ f2 sH[4 * 4 * kLanes];      // spectrum [block][reg][lane] (re, im), 8 KB
+  for (int i = threadIdx.x; i < 3 * 2 * 3 * kLanes; i += blockDim.x) {
+    const int idx = i / kLanes, ln = i % kLanes;
+    u4 w;
+#pragma unroll
+    for (int k = 0; k < 4; k++) w[k] = a.opsm[(idx * 4 + k) * kLanes + ln];
+    sF[i] = w;
+  }
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) sH[i] = f2{(float)a.H[i], (float)a.H[1024 + i]};
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, j = lane & 15;
+  const int64_t W = (int64_t)gridDim.x * 4;
+  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+  for (int64_t tp = gw; tp < a.pairs; tp += W) {
+    const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
+    double xa[16], xb[16];
+    gather32(a, pc, q, j, xa, xb);
+    const double cA = lane0(xa[0]), cB = lane0(xb[0]);
+    Blk4 zr, zi, orr, oi;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        zr.b[b][r] = (float)(xa[b * 4 + r] - cA);
+        zi.b[b][r] = (float)(xb[b * 4 + r] - cB);
+      }
+    stage32<false, NTERMS>(zr, zi, sF, lane, orr, oi);
+    stage32<false, NTERMS>(orr, oi, sF, lane, zr, zi);  // Y[k0][k1], block order
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const f2 h = sH[(b * 4 + v) * kLanes + lane];
+        const float yr = zr.b[b][v], yi = zi.b[b][v];
+        zr.b[b][v] = yr * h[0] - yi * h[1];
+        zi.b[b][v] = yr * h[1] + yi * h[0];
+      }
+    stage32<true, NTERMS>(zr, zi, sF, lane, orr, oi);
+    stage32<true, NTERMS>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
+    store32(a, pc, q, j, zr, zi, cA * a.dc_gain, cB * a.dc_gain);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 3D 16x16x16 tiles.  One workgroup (4 waves) per tile pair; the complex window (tile A real, tile B
+// imaginary) sits in LDS with a padded pitch, idx(x, y, z) = (16 x + y) * 17 + z.  Three phases:
+//   1. wave w, slices a = 4w..4w+3: contract axis 2 then axis 1 in registers  -> Y[a][k2][k1]
+//   2. wave w, k2 = 4w..4w+3: contract axis 0, multiply by H, inverse axis 1, inverse axis 0 -> W[k2][n1][n0]
+//   3. wave w, slices n0 = 4w..4w+3: inverse axis 2; lanes now run along axis 2 -> coalesced store
+// Every stage is the same accumulator-chained A^T F step as in 2D; only the re-slicing between phases
+// goes through LDS.  rowmap(q, x) is the row a lane group q holds in register x: 4x + q for FP64
+// accumulators, 4q + x for FP32 accumulators (the MFMA output orders measured on gfx950).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int lds3(int x, int y, int z) { return (16 * x + y) * 17 + z; }
+
+template <int PREC>
+__device__ __forceinline__ int rowmap(int q, int x) {
+  return PREC == 0 ? 4 * x + q : 4 * q + x;
+}
+
+template <int PREC, int NTERMS>
+struct Prec3;
+
+template <int NTERMS>
+struct Prec3<0, NTERMS> {  // FP64
+  typedef double V;
+  typedef d4 V4;
+  typedef OpsAxis Ops;
+  struct C2 {
+    double re, im;
+  };
+  template <bool INV>
+  static __device__ __forceinline__ void stage(const V4& ar, const V4& ai, const Ops& f, V4& outr, V4& outi) {
+    double cr[4], ci[4];
+    to_chunks(ar, cr);
+    to_chunks(ai, ci);
+    cstage<INV>(cr, ci, f, outr, outi);
+  }
+  static __device__ __forceinline__ void load_ops(Ops& f, const TileArgs& a, int lane) { ffsk::load_ops(f, a.ops, 0, lane); }
+};
+
+template <int NTERMS>
+struct Prec3<PREC_BF16, NTERMS> {
+  typedef float V;
+  typedef f4 V4;
+  typedef OpsBF16<NTERMS> Ops;
+  struct C2 {
+    float re, im;
+  };
+  template <bool INV>
+  static __device__ __forceinline__ void stage(const V4& ar, const V4& ai, const Ops& f, V4& outr, V4& outi) {
+    mstage<INV, NTERMS>(ar, ai, f, outr, outi);
+  }
+  static __device__ __forceinline__ void load_ops(Ops& f, const TileArgs& a, int lane) { load_ops_mixed(f, a.opsm, 0, lane); }
+};
+
+template <int NTERMS>
+struct Prec3<PREC_F32, NTERMS> {
+  typedef float V;
+  typedef f4 V4;
+  typedef OpsF32 Ops;
+  struct C2 {
+    float re, im;
+  };
+  template <bool INV>
+  static __device__ __forceinline__ void stage(const V4& ar, const V4& ai, const Ops& f, V4& outr, V4& outi) {
+    mstage<INV, NTERMS>(ar, ai, f, outr, outi);
+  }
+  static __device__ __forceinline__ void load_ops(Ops& f, const TileArgs& a, int lane) { load_ops_mixed(f, a.opsm, 0, lane); }
+};
+
+template <int PREC, int NTERMS>
+__global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
+  typedef Prec3<PREC, NTERMS> P;
+  typedef typename P::V V;
+  typedef typename P::V4 V4;
+  typedef typename P::C2 C2;
+  __shared__ C2 sX[16 * 16 * 17];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int q = lane >> 4, j = lane & 15;
+  typename P::Ops f;
+  P::load_ops(f, a, lane);
+  const bool mixed = PREC != 0;
+  const int64_t pairs2 = (a.tiles2 + 1) >> 1;
+
+  for (int64_t tp = xcd_logical_block(blockIdx.x, gridDim.x); tp < a.pairs; tp += gridDim.x) {
+    const int64_t T0 = tp / (a.tiles1 * pairs2);
+    const int64_t rem = tp - T0 * (a.tiles1 * pairs2);
+    const int64_t T1 = rem / pairs2, T2 = (rem - T1 * pairs2) * 2;
+    const bool hasB = T2 + 1 < a.tiles2;
+    const int64_t o0 = T0 * a.V0, o1 = T1 * a.V1, o2A = T2 * a.V2, o2B = o2A + a.V2;
+    const int64_t w0 = o0 - a.h0, w1 = o1 - a.h1, w2A = o2A - a.h2, w2B = o2B - a.h2;
+    const bool interior = w0 >= 0 && w0 + 16 <= a.n0 && w1 >= 0 && w1 + 16 <= a.n1 && w2A >= 0 &&
+                          (hasB ? w2B : w2A) + 16 <= a.n2;
+    // DC offsets (mixed forms): the window's first sample, a wave-uniform load
+    double cA = 0.0, cB = 0.0;
+    if (mixed) {
+      const int64_t base = (wrap_near(w0, a.n0) * a.n1 + wrap_near(w1, a.n1)) * a.n2;
+      cA = a.src[base + wrap_near(w2A, a.n2)];
+      cB = hasB ? a.src[base + wrap_near(w2B, a.n2)] : 0.0;
+    }
+    // ---- fill: thread (y = tid / 16, z = tid % 16) loads column z of row y in every slice x ----
+    {
+      const int y = threadIdx.x >> 4, z = threadIdx.x & 15;
+      const int64_t r1 = interior ? w1 + y : wrap_near(w1 + y, a.n1);
+      const int64_t cAz = interior ? w2A + z : wrap_near(w2A + z, a.n2);
+      const int64_t cBz = interior ? w2B + z : wrap_near(w2B + z, a.n2);
+#pragma unroll 4
+      for (int x = 0; x < 16; x++) {
+        const int64_t r0 = interior ? w0 + x : wrap_near(w0 + x, a.n0);
+        const double* row = a.src + (r0 * a.n1 + r1) * a.n2;
+        const double va = row[cAz];
+        const double vb = hasB ? row[cBz] : 0.0;
+        C2 c;
+        c.re = (V)(va - cA);
+        c.im = (V)(hasB ? vb - cB : 0.0);
+        sX[lds3(x, y, z)] = c;
+      }
+    }
+    __syncthreads();
+    // ---- phase 1: slices a = 4w + s; contract axis 2, then axis 1 ----
+#pragma unroll 1
+    for (int s = 0; s < 4; s++) {
+      const int sl = 4 * w + s;
+      V4 ar, ai;
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        const C2 c = sX[lds3(sl, j, rowmap<PREC>(q, x))];
+        ar[x] = c.re;
+        ai[x] = c.im;
+      }
+      V4 br, bi;
+      P::template stage<false>(ar, ai, f, br, bi);  // [n1 = row][k2 = lane]
+      P::template stage<false>(br, bi, f, ar, ai);  // [k2 = row][k1 = lane]
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        C2 c;
+        c.re = ar[v];
+        c.im = ai[v];
+        sX[lds3(sl, rowmap<PREC>(q, v), j)] = c;
+      }
+    }
+    __syncthreads();
+    // ---- phase 2: k2 = 4w + s; all reads first (the writes reuse the same LDS) ----
+    V4 pr[4], pi[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int k2 = 4 * w + s;
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        const C2 c = sX[lds3(rowmap<PREC>(q, x), k2, j)];  // Y[k0 row][k2][k1 = lane]
+        pr[s][x] = c.re;
+        pi[s][x] = c.im;
+      }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int s = 0; s < 4; s++) {
+      const int k2 = 4 * w + s;
+      V4 br, bi;
+      P::template stage<false>(pr[s], pi[s], f, br, bi);  // [k1 = row][k0 = lane]
+      // H entry (k2, v, lane): k0 = j, k1 = rowmap(q, v), k2
+      const double* H = a.H + (int64_t)k2 * 4 * kLanes + lane;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const V hr = (V)H[v * kLanes], hi = (V)H[4096 + v * kLanes];
+        const V yr = br[v], yi = bi[v];
+        br[v] = yr * hr - yi * hi;
+        bi[v] = yr * hi + yi * hr;
+      }
+      V4 cr, ci;
+      P::template stage<true>(br, bi, f, cr, ci);  // [k0 = row][n1 = lane]
+      P::template stage<true>(cr, ci, f, br, bi);  // [n1 = row][n0 = lane]
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        C2 c;
+        c.re = br[v];
+        c.im = bi[v];
+        sX[lds3(k2, rowmap<PREC>(q, v), j)] = c;  // W[k2][n1][n0]
+      }
+    }
+    __syncthreads();
+    // ---- phase 3: slices n0 = 4w + s; inverse axis 2; store ----
+#pragma unroll 1
+    for (int s = 0; s < 4; s++) {
+      const int n0 = 4 * w + s;
+      V4 ar, ai;
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        const C2 c = sX[lds3(rowmap<PREC>(q, x), j, n0)];  // W[k2 row][n1 = lane][n0]
+        ar[x] = c.re;
+        ai[x] = c.im;
+      }
+      V4 br, bi;
+      P::template stage<true>(ar, ai, f, br, bi);  // [n1 = row][n2 = lane]
+      const int64_t y0 = o0 + n0 - a.h0;
+      if (n0 >= a.h0 && n0 < a.h0 + a.V0 && y0 < a.n0) {
+        const bool colok = j >= a.h2 && j < a.h2 + a.V2;
+        const int64_t zA = o2A + j - a.h2, zB = o2B + j - a.h2;
+        const bool okA = colok && zA < a.n2, okB = colok && hasB && zB < a.n2;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int n1 = rowmap<PREC>(q, v);
+          const int64_t y1 = o1 + n1 - a.h1;
+          if (n1 >= a.h1 && n1 < a.h1 + a.V1 && y1 < a.n1) {
+            double* row = a.dst + (y0 * a.n1 + y1) * a.n2;
+            if (okA) row[zA] = (double)br[v] + cA * a.dc_gain;
+            if (okB) row[zB] = (double)bi[v] + cB * a.dc_gain;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Plan-time generation (FP64 on the GPU).
 // ---------------------------------------------------------------------------------------------
 __global__ void k_gen_operands(double* ops, int32_t L0, int32_t L1, int32_t L2) {
@@ -644,40 +1043,67 @@ __global__ void k_gen_operands_mixed(uint32_t* t, int32_t L0, int32_t L1, int32_
   }
 }
 
-// K(k) = sum_taps w[a]// K(k) = sum_taps w[a] exp(-2 pi i sum_ax (d_ax(a) k_ax mod L_ax)/L_ax),  d = a (ref) or r - a (centred);
+// 32-point BF16 B operands for KIND_2D32 (layout in ffs_internal.h); one thread per (arr, Kb, term, lane).
+__global__ void k_gen_operands32(uint32_t* t) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 3 * 2 * 3 * kLanes) return;
+  const int lane = idx % kLanes, term = (idx / kLanes) % 3, Kb = (idx / (3 * kLanes)) % 2, arr = idx / (6 * kLanes);
+  const int q = lane >> 4, j = lane & 15;
+  for (int w = 0; w < 4; w++) {
+    uint32_t word = 0;
+    for (int half = 0; half < 2; half++) {
+      const int e = 2 * w + half;
+      double fr, fi;
+      dft_entry(16 * (e >> 2) + 4 * q + (e & 3), 16 * Kb + j, 32, fr, fi);
+      const double v = arr == 0 ? fr : (arr == 1 ? fi : -fi);
+      __bf16 h, m, l;
+      split3_bf16(v, h, m, l);
+      const __bf16 pick = term == 0 ? h : (term == 1 ? m : l);
+      word |= (uint32_t)__builtin_bit_cast(uint16_t, pick) << (16 * half);
+    }
+    t[((((arr * 2 + Kb) * 3 + term) * 4) + w) * kLanes + lane] = word;
+  }
+}
+
+// K(k) = sum_taps w[a]// K(k) = sum_taps w[a]// K(k) = sum_taps w[a] exp(-2 pi i sum_ax (d_ax(a) k_ax mod L_ax)/L_ax),  d = a (ref) or r - a (centred);
 // H = K^t / prod(L) by repeated squaring in complex FP64.  `kind` fixes which frequency sits at
 // entry e of the family layout.
 __global__ void k_gen_spectrum(double* H, int kind, int layout, int32_t L0, int32_t L1, int32_t L2,
                                const double* taps, int32_t K, int32_t dims, int32_t convention, int32_t radius,
                                int64_t t) {
-  const int64_t len = kind == KIND_3D16 ? 4096 : 256;
+  const int64_t len = kind == KIND_3D16 ? 4096 : (kind == KIND_2D32 ? 1024 : 256);
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= len) return;
   const int lane = (int)(e % kLanes);
-  const int q = lane >> 4, j = lane & 15;
+  const int q = lane >> 4;
+  int j = lane & 15;
   int v, kb = 0;
-  if (kind == KIND_3D16) {
+  if (kind == KIND_3D16 || kind == KIND_2D32) {
     v = (int)((e / kLanes) % 4);
     kb = (int)(e / (4 * kLanes));
   } else {
     v = (int)(e / kLanes);
   }
-  const int krow = layout == 0 ? q + 4 * v : 4 * q + v;  // frequency on the row axis of the family layout
+  int krow = layout == 0 ? q + 4 * v : 4 * q + v;  // frequency on the row axis of the family layout
+  if (kind == KIND_2D32) {  // block kb = 2 R + C: rows 16 R + 4q + v, columns 16 C + j
+    krow = 16 * (kb >> 1) + 4 * q + v;
+    j += 16 * (kb & 1);
+  }
   double kr = 0.0, ki = 0.0;
   bool zero = false;
   int kax[3] = {0, 0, 0};
   int Lax[3] = {1, 1, 1};
   int nax = 0;
   int64_t k1d = 0;
-  if (kind == KIND_2D16) {
+  if (kind == KIND_2D16 || kind == KIND_2D32) {
     kax[0] = krow; kax[1] = j; Lax[0] = L0; Lax[1] = L1; nax = 2;
     zero = krow >= L0 || j >= L1;
   } else if (kind == KIND_1DPFA) {
     zero = krow >= 15;
     k1d = (15 * j + 16 * krow) % 240;
     nax = 1;
-  } else {
-    kax[0] = krow; kax[1] = kb; kax[2] = j; Lax[0] = L0; Lax[1] = L1; Lax[2] = L2; nax = 3;
+  } else {  // 3D entry (k2 block, reg v, lane): k0 = lane % 16, k1 = rowmap(q, v), k2 = block
+    kax[0] = j; kax[1] = krow; kax[2] = kb; Lax[0] = L0; Lax[1] = L1; Lax[2] = L2; nax = 3;
   }
   if (!zero) {
     int64_t ntaps = 1;
@@ -738,9 +1164,14 @@ hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hi
   return hipGetLastError();
 }
 
+hipError_t gen_operands32(uint32_t* ops32, hipStream_t s) {
+  hipLaunchKernelGGL(k_gen_operands32, dim3((3 * 2 * 3 * kLanes + 255) / 256), dim3(256), 0, s, ops32);
+  return hipGetLastError();
+}
+
 hipError_t gen_spectrum(double* H, int kind, int layout, const int32_t* axis_len, const double* d_taps, int32_t K,
                         int32_t dims, int32_t convention, int32_t radius, int64_t t, hipStream_t s) {
-  const int len = kind == KIND_3D16 ? 4096 : 256;
+  const int len = (int)spectrum_len(kind);
   hipLaunchKernelGGL(k_gen_spectrum, dim3((len + 255) / 256), dim3(256), 0, s, H, kind, layout, axis_len[0],
                      axis_len[1], axis_len[2], d_taps, K, dims, convention, radius, t);
   return hipGetLastError();
@@ -748,9 +1179,24 @@ hipError_t gen_spectrum(double* H, int kind, int layout, const int32_t* axis_len
 
 hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus) {
   if (a.pairs <= 0) return hipSuccess;
+  if (kind == KIND_3D16) {  // one workgroup per tile pair, two resident per CU (68 KB LDS each for FP64)
+    const int64_t blocks3 = std::min<int64_t>(a.pairs, (int64_t)num_cus * (a.precision == 0 ? 2 : 3));
+    const dim3 g3((unsigned)blocks3), b3(256);
+    if (a.precision == PREC_BF16)
+      hipLaunchKernelGGL((k_tile3d<PREC_BF16, 3>), g3, b3, 0, s, a);
+    else if (a.precision == PREC_BF16X2)
+      hipLaunchKernelGGL((k_tile3d<PREC_BF16, 2>), g3, b3, 0, s, a);
+    else if (a.precision == PREC_F32)
+      hipLaunchKernelGGL((k_tile3d<PREC_F32, 1>), g3, b3, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_tile3d<0, 1>), g3, b3, 0, s, a);
+    return hipGetLastError();
+  }
   const int64_t waves_needed = a.pairs;
   int64_t blocks = (waves_needed + 3) / 4;
-  const int64_t cap = (int64_t)num_cus * 8;  // 8 workgroups (32 waves) per CU at most, grid-stride beyond
+  // 8 workgroups (32 waves) per CU at most, grid-stride beyond; the 32-point kernel stages its tables
+  // in LDS per workgroup and runs 2 waves per SIMD, so it launches exactly the resident workgroups.
+  const int64_t cap = (int64_t)num_cus * (kind == KIND_2D32 ? 2 : 8);
   if (blocks > cap) blocks = cap;
   const dim3 g((unsigned)blocks), b(256);
   if (kind == KIND_2D16) {
@@ -771,6 +1217,13 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
       hipLaunchKernelGGL((k_tile16_mixed<KIND_1DPFA, PREC_F32, 1>), g, b, 0, s, a);
     else
       hipLaunchKernelGGL(k_tile16_f64<KIND_1DPFA>, g, b, 0, s, a);
+  } else if (kind == KIND_2D32) {
+    if (a.precision == PREC_BF16)
+      hipLaunchKernelGGL(k_tile32_bf16<3>, g, b, 0, s, a);
+    else if (a.precision == PREC_BF16X2)
+      hipLaunchKernelGGL(k_tile32_bf16<2>, g, b, 0, s, a);
+    else
+      return hipErrorInvalidValue;
   } else {
     return hipErrorInvalidValue;
   }

@@ -97,3 +97,48 @@ def test_mixed_deterministic(ffs, oracle, torch_cuda):
     torch.cuda.synchronize()
     plan.close()
     assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("form", ["bf16", "bf16x2"])
+@pytest.mark.parametrize("tile", [16, 32])
+@pytest.mark.parametrize("size", [(37, 53), (100, 64), (7, 300)])
+def test_2d_bf16_tiles(ffs, oracle, torch_cuda, form, tile, size):
+    _, r, w, _ = ffs.named_stencil("Box2D25P", seed=31)
+    x = oracle.uniform(55 + size[0], size)
+    for steps in (1, 4, 11):
+        got, info = _run(ffs, torch_cuda, x, r, w, steps, precision=form, tile=tile)
+        assert info["tile"][:2] == [tile, tile]
+        ref = oracle.sweep(x, r, w, steps)
+        _, rel, _ = oracle.errors(got, ref)
+        tol = TOL_MIXED if form == "bf16" else 2e-5  # the 2-term split is only 1e-6-accurate on smoothed fields
+        assert rel <= tol, (form, tile, size, steps, info, rel)
+
+
+@pytest.mark.parametrize("form", ["bf16", "f32"])
+@pytest.mark.parametrize("shape", ["Heat-3D", "Box3D27P"])
+@pytest.mark.parametrize("size", [(16, 16, 16), (11, 13, 17), (40, 24, 30)])
+def test_3d_mixed_named(ffs, oracle, torch_cuda, form, shape, size):
+    _, r, w, _ = ffs.named_stencil(shape, seed=17)
+    x = oracle.uniform(5000 + sum(size), size)
+    for steps in (1, 6):
+        got, info = _run(ffs, torch_cuda, x, r, w, steps, precision=form)
+        ref = oracle.sweep(x, r, w, steps)
+        _, rel, _ = oracle.errors(got, ref)
+        assert rel <= TOL_MIXED, (form, shape, size, steps, info, rel)
+
+
+@pytest.mark.parametrize("form", ["bf16", "f32"])
+def test_3d_mixed_sinusoid_field(ffs, oracle, torch_cuda, form):
+    # the 3D workload's field: a sum of random-phase integer-wavevector sinusoids plus small noise
+    n = 32
+    rng = np.random.default_rng(3)
+    idx = np.indices((n, n, n)).astype(np.float64)
+    x = np.zeros((n, n, n))
+    for _ in range(8):
+        k = rng.integers(1, 5, size=3)
+        x += rng.random() / 8 * np.cos(2 * np.pi * (k[0] * idx[0] + k[1] * idx[1] + k[2] * idx[2]) / n + 2 * np.pi * rng.random())
+    x += 1e-3 * rng.random((n, n, n))
+    _, r, w, _ = ffs.named_stencil("Box3D27P", seed=0x5EED0003)
+    got, info = _run(ffs, torch_cuda, x, r, w, 16, precision=form)
+    ref = oracle.sweep(x, r, w, 16)
+    assert oracle.errors(got, ref)[1] <= TOL_MIXED, info

@@ -123,3 +123,27 @@ def test_run_host_matches_device(ffs, oracle, torch_cuda):
     plan.close()
     ref = oracle.sweep(x, r, w, 6)
     assert oracle.errors(got, ref)[1] <= TOL_FP64
+
+
+@pytest.mark.parametrize("shape", ["Heat-3D", "Box3D27P"])
+@pytest.mark.parametrize("size", [(16, 16, 16), (11, 13, 17), (20, 9, 33), (5, 5, 5), (40, 24, 30)])
+@pytest.mark.parametrize("steps", [1, 2, 5])
+def test_3d_fp64_named(ffs, oracle, torch_cuda, shape, size, steps):
+    _, r, w, _ = ffs.named_stencil(shape, seed=13)
+    x = oracle.uniform(3000 + sum(size), size)
+    got, info = _run(ffs, torch_cuda, x, r, w, steps)
+    ref = oracle.sweep(x, r, w, steps)
+    _, rel, _ = oracle.errors(got, ref)
+    assert rel <= TOL_FP64, (shape, size, steps, info, rel)
+
+
+@pytest.mark.parametrize("convention", ["ref", "centred"])
+@pytest.mark.parametrize("fuse", [1, 3])
+def test_3d_fp64_fuse_convention(ffs, oracle, torch_cuda, convention, fuse):
+    rng = np.random.default_rng(70 + fuse)
+    w = rng.random(27)
+    w /= w.sum()
+    x = oracle.uniform(71, (18, 22, 26))
+    got, info = _run(ffs, torch_cuda, x, 1, w, 2 * fuse + 1, convention=convention, fuse=fuse)
+    ref = oracle.sweep(x, 1, w, 2 * fuse + 1, convention=0 if convention == "ref" else 1)
+    assert oracle.errors(got, ref)[1] <= TOL_FP64, info
