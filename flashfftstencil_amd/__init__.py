@@ -123,6 +123,10 @@ def load_library() -> ctypes.CDLL:
         "ffs_run_block": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p],
         "ffs_format_csv": [ctypes.c_char_p, ctypes.c_int32, P(ctypes.c_int64), ctypes.c_int64, ctypes.c_double,
                            ctypes.c_char_p, ctypes.c_int32],
+        "ffs_batch_create": [P(Problem), ctypes.c_int32, P(ctypes.c_void_p)],
+        "ffs_batch_run": [ctypes.c_void_p, P(ctypes.c_void_p), P(ctypes.c_void_p), ctypes.c_void_p],
+        "ffs_batch_get_info": [ctypes.c_void_p, ctypes.c_int32, P(PlanInfo)],
+        "ffs_batch_destroy": [ctypes.c_void_p],
     }
     for name, argtypes in sigs.items():
         fn = getattr(lib, name)
@@ -248,6 +252,62 @@ class Plan:
 
     def __exit__(self, *exc):
         self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent 2D grids advanced together (one persistent launch per fused block, BF16 forms).
+
+    `problems` is a list of dicts with keys sizes, weights, radius, steps and optionally precision,
+    convention, fuse (tile is always 32)."""
+
+    def __init__(self, problems, precision: str = "bf16"):
+        lib = load_library()
+        n = len(problems)
+        arr = (Problem * n)()
+        self._w = []
+        self.sizes = []
+        for i, pr in enumerate(problems):
+            p = arr[i]
+            _check(lib.ffs_problem_init(ctypes.byref(p)))
+            w = np.ascontiguousarray(np.asarray(pr["weights"], dtype=np.float64).ravel())
+            self._w.append(w)
+            p.dims = 2
+            p.radius = int(pr["radius"])
+            p.sizes[0], p.sizes[1] = int(pr["sizes"][0]), int(pr["sizes"][1])
+            p.weights = _dptr(w)
+            p.steps = int(pr["steps"])
+            prec = pr.get("precision", precision)
+            p.precision = PRECISIONS[prec] if isinstance(prec, str) else int(prec)
+            conv = pr.get("convention", "ref")
+            p.convention = CONVENTIONS[conv] if isinstance(conv, str) else int(conv)
+            p.fuse = int(pr.get("fuse", 0))
+            self.sizes.append((int(pr["sizes"][0]), int(pr["sizes"][1])))
+        h = ctypes.c_void_p()
+        _check(lib.ffs_batch_create(arr, n, ctypes.byref(h)))
+        self._h = h
+        self.count = n
+
+    def info(self, i: int) -> dict:
+        inf = PlanInfo()
+        _check(load_library().ffs_batch_get_info(self._h, i, ctypes.byref(inf)))
+        return inf.as_dict()
+
+    def run(self, d_ins, d_outs, stream=None) -> None:
+        n = self.count
+        ins = (ctypes.c_void_p * n)(*[_ptr(x) for x in d_ins])
+        outs = (ctypes.c_void_p * n)(*[_ptr(x) for x in d_outs])
+        _check(load_library().ffs_batch_run(self._h, ins, outs, ctypes.c_void_p(_stream_handle(stream))))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load_library().ffs_batch_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:

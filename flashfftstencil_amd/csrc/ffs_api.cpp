@@ -89,6 +89,22 @@ struct ffs_plan {
   double* g_out = nullptr;
 };
 
+struct ffs_batch {
+  std::vector<ffs_plan*> plans;
+  int32_t precision = 0;
+  int dev = 0;
+  int num_cus = 256;
+  int64_t rounds = 0;
+  ffsk::TileArgs* d_items = nullptr;  // [round][grid]
+  int64_t* d_prefix = nullptr;        // [round][grid + 1]
+  std::vector<int32_t> active;        // grids with a block in each round
+  std::vector<const double*> g_in;
+  std::vector<double*> g_out;
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
 namespace {
 
 ffs_status validate(const ffs_problem* p, int64_t* npoints) {
@@ -583,6 +599,146 @@ ffs_status ffs_run_host(ffs_plan* pl, const double* h_in, double* h_out) {
   }
   if (st != FFS_OK) return st;
   if (e != hipSuccess) return fail(FFS_ERR_HIP, "run_host failed: %s", hipGetErrorString(e));
+  return FFS_OK;
+}
+
+ffs_status ffs_batch_create(const ffs_problem* problems, int32_t count, ffs_batch** out) {
+  if (!out) return fail(FFS_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  if (!problems || count <= 0) return fail(FFS_ERR_INVALID_ARG, "batch needs at least one problem");
+  const int32_t prec = problems[0].precision;
+  for (int32_t i = 0; i < count; i++) {
+    if (problems[i].dims != 2) return fail(FFS_ERR_UNSUPPORTED, "batch problem %d: only 2D grids are batched", i);
+    if (problems[i].precision != prec)
+      return fail(FFS_ERR_INVALID_ARG, "batch problem %d: all problems must share one precision", i);
+    if (prec != FFS_PREC_MIXED_BF16 && prec != FFS_PREC_MIXED_BF16X2)
+      return fail(FFS_ERR_UNSUPPORTED, "batches run the BF16 mixed forms (32-point tiles)");
+    if (problems[i].tile != 0 && problems[i].tile != 32)
+      return fail(FFS_ERR_UNSUPPORTED, "batch problem %d: tile must be 32", i);
+    int64_t np = 0;
+    ffs_status st = validate(&problems[i], &np);
+    if (st != FFS_OK) return st;
+  }
+  ffs_batch* b = new (std::nothrow) ffs_batch();
+  if (!b) return fail(FFS_ERR_OUT_OF_MEMORY, "host allocation failed");
+  b->precision = prec;
+  auto bail = [&](ffs_status st) {
+    ffs_batch_destroy(b);
+    return st;
+  };
+  for (int32_t i = 0; i < count; i++) {
+    ffs_problem p = problems[i];
+    p.tile = 32;
+    ffs_plan* pl = nullptr;
+    ffs_status st = ffs_plan_create(&p, &pl);
+    if (st != FFS_OK) return bail(st);
+    b->plans.push_back(pl);
+    b->rounds = std::max(b->rounds, pl->nblocks);
+  }
+  b->dev = b->plans[0]->dev;
+  b->num_cus = b->plans[0]->num_cus;
+  b->active.assign((size_t)b->rounds, 0);
+  hipError_t e = hipSuccess;
+  if (b->rounds > 0) {
+    e = hipMalloc(&b->d_items, sizeof(ffsk::TileArgs) * b->rounds * count);
+    if (e == hipSuccess) e = hipMalloc(&b->d_prefix, sizeof(int64_t) * b->rounds * (count + 1));
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    fail(FFS_ERR_HIP, "batch set-up failed: %s", hipGetErrorString(e));
+    return bail(FFS_ERR_HIP);
+  }
+  *out = b;
+  return FFS_OK;
+}
+
+ffs_status ffs_batch_get_info(const ffs_batch* b, int32_t index, ffs_plan_info* info) {
+  if (!b || index < 0 || index >= (int32_t)b->plans.size()) return fail(FFS_ERR_INVALID_ARG, "bad batch index");
+  return ffs_plan_get_info(b->plans[index], info);
+}
+
+ffs_status ffs_batch_run(ffs_batch* b, const double* const* d_in, double* const* d_out, void* stream) {
+  if (!b || !d_in || !d_out) return fail(FFS_ERR_INVALID_ARG, "NULL argument");
+  const int32_t n = (int32_t)b->plans.size();
+  for (int32_t i = 0; i < n; i++) {
+    if (!d_in[i] || !d_out[i]) return fail(FFS_ERR_INVALID_ARG, "grid %d: NULL pointer", i);
+    if (d_in[i] == d_out[i]) return fail(FFS_ERR_INVALID_ARG, "grid %d: d_in must not alias d_out", i);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  FFS_HIP(hipSetDevice(b->dev));
+  const bool same = b->gexec && std::equal(b->g_in.begin(), b->g_in.end(), d_in) &&
+                    std::equal(b->g_out.begin(), b->g_out.end(), d_out) && (int32_t)b->g_in.size() == n;
+  if (!same && b->rounds > 0) {
+    // per-round work lists: grid g's block r reads its previous output and writes d_out or the plan's
+    // ping-pong grid so that its last block lands in d_out
+    std::vector<ffsk::TileArgs> items((size_t)(b->rounds * n));
+    std::vector<int64_t> prefix((size_t)(b->rounds * (n + 1)));
+    for (int64_t r = 0; r < b->rounds; r++) {
+      int32_t k = 0;
+      int64_t acc = 0;
+      for (int32_t g = 0; g < n; g++) {
+        ffs_plan* pl = b->plans[g];
+        const int64_t nb = pl->nblocks;
+        if (r >= nb) continue;
+        const double* src = d_in[g];
+        double* dst = nullptr;
+        for (int64_t q = 0; q <= r; q++) {
+          dst = ((nb - 1 - q) % 2 == 0) ? d_out[g] : pl->d_scratch;
+          if (q < r) src = dst;
+        }
+        const bool tail = (r == nb - 1) && pl->t_tail > 0;
+        items[(size_t)(r * n + k)] = make_args(pl, src, dst, tail);
+        prefix[(size_t)(r * (n + 1) + k)] = acc;
+        acc += pl->pairs;
+        k++;
+      }
+      prefix[(size_t)(r * (n + 1) + k)] = acc;
+      b->active[(size_t)r] = k;
+    }
+    FFS_HIP(hipMemcpy(b->d_items, items.data(), sizeof(ffsk::TileArgs) * items.size(), hipMemcpyHostToDevice));
+    FFS_HIP(hipMemcpy(b->d_prefix, prefix.data(), sizeof(int64_t) * prefix.size(), hipMemcpyHostToDevice));
+    if (b->gexec) {
+      FFS_HIP(hipGraphExecDestroy(b->gexec));
+      b->gexec = nullptr;
+    }
+    if (b->graph) {
+      FFS_HIP(hipGraphDestroy(b->graph));
+      b->graph = nullptr;
+    }
+    FFS_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    hipError_t le = hipSuccess;
+    for (int64_t r = 0; r < b->rounds && le == hipSuccess; r++)
+      le = ffsk::launch_batch(b->d_items + r * n, b->d_prefix + r * (n + 1), b->active[(size_t)r], b->precision,
+                              b->cap_stream, b->num_cus);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(b->cap_stream, &g);
+    if (le != hipSuccess || ee != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(FFS_ERR_HIP, "batch capture failed: %s", hipGetErrorString(le != hipSuccess ? le : ee));
+    }
+    b->graph = g;
+    FFS_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    b->g_in.assign(d_in, d_in + n);
+    b->g_out.assign(d_out, d_out + n);
+  }
+  for (int32_t g = 0; g < n; g++)  // grids with T = 0 are copies
+    if (b->plans[g]->nblocks == 0)
+      FFS_HIP(hipMemcpyAsync(d_out[g], d_in[g], sizeof(double) * b->plans[g]->npoints, hipMemcpyDeviceToDevice, s));
+  if (b->gexec) FFS_HIP(hipGraphLaunch(b->gexec, s));
+  return FFS_OK;
+}
+
+ffs_status ffs_batch_destroy(ffs_batch* b) {
+  if (!b) return FFS_OK;
+  (void)hipSetDevice(b->dev);
+  (void)hipDeviceSynchronize();
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  if (b->d_items) (void)hipFree(b->d_items);
+  if (b->d_prefix) (void)hipFree(b->d_prefix);
+  for (ffs_plan* pl : b->plans) release(pl);
+  delete b;
   return FFS_OK;
 }
 

@@ -699,53 +699,101 @@ __device__ __forceinline__ void store32(const TileArgs& a, const PairCoord& pc, 
   }
 }
 
-template <int NTERMS>
-__global__ __launch_bounds__(256, 2) void k_tile32_bf16(TileArgs a) {
-  __shared__ u4 sF[3 * 2 * 3 * kLanes];  // B operands, 18 KB
-  __shared__ f2 sH[4 * 4 * kLanes];      // spectrum [block][reg][lane] (re, im), 8 KB
+// One tile pair of the 32-point family.  The spectrum comes from LDS (single-grid launch) or from the
+// grid's FP64 table in global memory (batch launch, where every grid has its own stencil and depth).
+template <int NTERMS, bool H_FROM_LDS>
+__device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const u4* __restrict__ sF,
+                                            const f2* __restrict__ sH, int lane, int q, int j) {
+  const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
+  double xa[16], xb[16];
+  gather32(a, pc, q, j, xa, xb);
+  const double cA = lane0(xa[0]), cB = lane0(xb[0]);
+  Blk4 zr, zi, orr, oi;
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      zr.b[b][r] = (float)(xa[b * 4 + r] - cA);
+      zi.b[b][r] = (float)(xb[b * 4 + r] - cB);
+    }
+  stage32<false, NTERMS>(zr, zi, sF, lane, orr, oi);
+  stage32<false, NTERMS>(orr, oi, sF, lane, zr, zi);  // Y[k0][k1], block order
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int e = (b * 4 + v) * kLanes + lane;
+      const float hr = H_FROM_LDS ? sH[e][0] : (float)a.H[e];
+      const float hi = H_FROM_LDS ? sH[e][1] : (float)a.H[1024 + e];
+      const float yr = zr.b[b][v], yi = zi.b[b][v];
+      zr.b[b][v] = yr * hr - yi * hi;
+      zi.b[b][v] = yr * hi + yi * hr;
+    }
+  stage32<true, NTERMS>(zr, zi, sF, lane, orr, oi);
+  stage32<true, NTERMS>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
+  store32(a, pc, q, j, zr, zi, cA * a.dc_gain, cB * a.dc_gain);
+}
+
+__device__ __forceinline__ void stage_ops32(u4* sF, const uint32_t* __restrict__ opsm) {
   for (int i = threadIdx.x; i < 3 * 2 * 3 * kLanes; i += blockDim.x) {
     const int idx = i / kLanes, ln = i % kLanes;
     u4 w;
 #pragma unroll
-    for (int k = 0; k < 4; k++) w[k] = a.opsm[(idx * 4 + k) * kLanes + ln];
+    for (int k = 0; k < 4; k++) w[k] = opsm[(idx * 4 + k) * kLanes + ln];
     sF[i] = w;
   }
+}
+
+template <int NTERMS>
+__global__ __launch_bounds__(256, 2) void k_tile32_bf16(TileArgs a) {
+  __shared__ u4 sF[3 * 2 * 3 * kLanes];  // B operands, 18 KB
+  __shared__ f2 sH[4 * 4 * kLanes];      // spectrum [block][reg][lane] (re, im), 8 KB
+  stage_ops32(sF, a.opsm);
   for (int i = threadIdx.x; i < 1024; i += blockDim.x) sH[i] = f2{(float)a.H[i], (float)a.H[1024 + i]};
   __syncthreads();
-
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair<NTERMS, true>(a, tp, sF, sH, lane, q, j);
+}
 
-  for (int64_t tp = gw; tp < a.pairs; tp += W) {
-    const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
-    double xa[16], xb[16];
-    gather32(a, pc, q, j, xa, xb);
-    const double cA = lane0(xa[0]), cB = lane0(xb[0]);
-    Blk4 zr, zi, orr, oi;
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        zr.b[b][r] = (float)(xa[b * 4 + r] - cA);
-        zi.b[b][r] = (float)(xb[b * 4 + r] - cB);
-      }
-    stage32<false, NTERMS>(zr, zi, sF, lane, orr, oi);
-    stage32<false, NTERMS>(orr, oi, sF, lane, zr, zi);  // Y[k0][k1], block order
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const f2 h = sH[(b * 4 + v) * kLanes + lane];
-        const float yr = zr.b[b][v], yi = zi.b[b][v];
-        zr.b[b][v] = yr * h[0] - yi * h[1];
-        zi.b[b][v] = yr * h[1] + yi * h[0];
-      }
-    stage32<true, NTERMS>(zr, zi, sF, lane, orr, oi);
-    stage32<true, NTERMS>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
-    store32(a, pc, q, j, zr, zi, cA * a.dc_gain, cB * a.dc_gain);
+// Multi-grid launch: one fused block of every grid still running, as one persistent launch over the
+// concatenated work list.  items[g] describes grid g's block (source, destination, depth, spectrum);
+// prefix[g] is the first work index of grid g (prefix[n] = total).  All items cost the same (one 32x32
+// tile pair), so waves take them in a fixed strided order: balanced and bitwise reproducible.
+template <int NTERMS>
+__global__ __launch_bounds__(256, 2) void k_batch32_bf16(const TileArgs* __restrict__ items,
+                                                         const int64_t* __restrict__ prefix, int32_t n) {
+  __shared__ u4 sF[3 * 2 * 3 * kLanes];
+  stage_ops32(sF, items[0].opsm);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, j = lane & 15;
+  const int64_t W = (int64_t)gridDim.x * 4;
+  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t total = prefix[n];
+  for (int64_t idx = gw; idx < total; idx += W) {
+    int lo = 0, hi = n;  // largest g with prefix[g] <= idx
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (prefix[mid] <= idx) lo = mid; else hi = mid;
+    }
+    const TileArgs a = items[lo];
+    tile32_pair<NTERMS, false>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
   }
+}
+
+hipError_t launch_batch(const TileArgs* d_items, const int64_t* d_prefix, int32_t n, int32_t precision, hipStream_t s,
+                        int num_cus) {
+  const dim3 g((unsigned)num_cus * 2), b(256);
+  if (precision == PREC_BF16)
+    hipLaunchKernelGGL(k_batch32_bf16<3>, g, b, 0, s, d_items, d_prefix, n);
+  else if (precision == PREC_BF16X2)
+    hipLaunchKernelGGL(k_batch32_bf16<2>, g, b, 0, s, d_items, d_prefix, n);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------------------------

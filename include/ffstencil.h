@@ -130,6 +130,18 @@ ffs_status ffs_run_host(ffs_plan* plan, const double* h_in, double* h_out);
  * d_out = S^t(d_in).  One kernel launch; used to time the tile kernel alone. */
 ffs_status ffs_run_block(ffs_plan* plan, const double* d_in, double* d_out, int32_t tail, void* stream);
 
+/* Batches of independent 2D grids (different sizes, stencils and T).  Block b of every grid that still
+ * has blocks runs in ONE persistent launch over the concatenated tile-pair list of those grids; each
+ * grid keeps its own fused depth, tail and spectrum.  All problems must be 2D and share one BF16 mixed
+ * precision (32-point tiles).  No reference counterpart (the reference runs one grid per process);
+ * it is the multi-grid workload's entry point.  d_in / d_out are host arrays of `count` device
+ * pointers; grid i advances problems[i].steps steps. */
+typedef struct ffs_batch ffs_batch;
+ffs_status ffs_batch_create(const ffs_problem* problems, int32_t count, ffs_batch** out);
+ffs_status ffs_batch_run(ffs_batch* batch, const double* const* d_in, double* const* d_out, void* stream);
+ffs_status ffs_batch_get_info(const ffs_batch* batch, int32_t index, ffs_plan_info* info);
+ffs_status ffs_batch_destroy(ffs_batch* batch);
+
 /* Reference-format report line:
  *   "FlashFFTStencil, <tag>, <d1>, <d2>, <d3>, <T>, <ms>, <GStencil/s>"
  * with leading 1s for unused dimensions and GStencil/s = points * T / ms / 1e6. */

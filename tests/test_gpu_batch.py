@@ -1,0 +1,54 @@
+"""GPU parity of the multi-grid batch (one persistent launch per fused block over all grids still
+running): every grid against the FP64 oracle (<= 1e-6 max-rel) and bitwise against the same grid run
+alone through the single-grid plan (same per-tile arithmetic, so identical results)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs(ffs, seed):
+    shapes = ["Heat-2D", "Star2D9P", "Box2D9P", "Box2D25P"]
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(7):
+        shape = shapes[i % 4]
+        _, r, w, _ = ffs.named_stencil(shape, seed=100 + i)
+        n0, n1 = (int(v) for v in rng.integers(5, 90, size=2))
+        steps = int(rng.choice([0, 1, 3, 7, 16, 25]))
+        specs.append(dict(shape=shape, radius=r, weights=w, sizes=(n0, n1), steps=steps))
+    specs.append(dict(shape="Box2D25P", radius=2, weights=ffs.named_stencil("Box2D25P", 9)[2], sizes=(130, 97), steps=40))
+    return specs
+
+
+@pytest.mark.parametrize("form", ["bf16", "bf16x2"])
+def test_batch_matches_oracle_and_single_plans(ffs, oracle, torch_cuda, form):
+    torch = torch_cuda
+    specs = _specs(ffs, 5)
+    xs = [oracle.uniform(200 + i, s["sizes"]) for i, s in enumerate(specs)]
+    d_in = [torch.from_numpy(x).cuda() for x in xs]
+    d_out = [torch.empty_like(d) for d in d_in]
+    batch = ffs.Batch(specs, precision=form)
+    batch.run(d_in, d_out)
+    batch.run(d_in, d_out)  # graph replay
+    torch.cuda.synchronize()
+    for i, s in enumerate(specs):
+        got = d_out[i].cpu().numpy()
+        ref = oracle.sweep(xs[i], s["radius"], s["weights"], s["steps"])
+        tol = 1e-6 if form == "bf16" else 2e-5
+        assert oracle.errors(got, ref)[1] <= tol, (i, s["shape"], s["sizes"], s["steps"], batch.info(i))
+        plan = ffs.Plan(2, s["sizes"], s["weights"], s["radius"], s["steps"], precision=form, tile=32)
+        single = torch.empty_like(d_in[i])
+        plan.run(d_in[i], single)
+        torch.cuda.synchronize()
+        plan.close()
+        assert torch.equal(single, d_out[i]), (i, s["shape"])
+    batch.close()
+
+
+def test_batch_rejects_mixed_precisions(ffs):
+    _, r, w, _ = ffs.named_stencil("Box2D9P")
+    specs = [dict(radius=r, weights=w, sizes=(32, 32), steps=2, precision="bf16"),
+             dict(radius=r, weights=w, sizes=(32, 32), steps=2, precision="bf16x2")]
+    with pytest.raises(ffs.FFSError):
+        ffs.Batch(specs)
