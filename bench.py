@@ -34,8 +34,49 @@ WORKLOADS = {
     "box2d25p": dict(shape="Box2D25P", sizes=(8192, 8192), T=256, seed=0x5EED0002, forms=("bf16", "fp64")),
     "box2d9p": dict(shape="Box2D9P", sizes=(8192, 8192), T=128, seed=0x5EED0001, forms=("fp64",)),
     "heat1d": dict(shape="Heat-1D", sizes=(1048576,), T=64, seed=0x5EED0000, forms=("fp64",)),
-    "box3d27p": dict(shape="Box3D27P", sizes=(512, 512, 512), T=32, seed=0x5EED0003, forms=("f32", "fp64")),
+    "box3d27p": dict(shape="Box3D27P", sizes=(512, 512, 512), T=32, seed=0x5EED0003, forms=("f32", "bf16", "fp64"),
+                     init="sinusoids"),
+    "batch96": dict(seed=0x5EED0004, forms=("bf16",)),
 }
+
+
+def sinusoid_field(ffs, seed, sizes):
+    """Sum of 8 sinusoids (amplitude uniform[0,1)/8, phase uniform[0, 2 pi), integer wavevectors uniform
+    in [1, 16]^3) plus 1e-3 uniform noise; all draws from SplitMix64(seed)."""
+    u = ffs.fill_uniform(seed, 8 * 5)
+    n0, n1, n2 = sizes
+    x = np.zeros(sizes)
+    i0 = np.arange(n0)[:, None, None] / n0
+    i1 = np.arange(n1)[None, :, None] / n1
+    i2 = np.arange(n2)[None, None, :] / n2
+    for m in range(8):
+        amp, ph = u[5 * m] / 8, 2 * np.pi * u[5 * m + 1]
+        k = [1 + int(16 * u[5 * m + 2 + a]) for a in range(3)]
+        x += amp * np.cos(2 * np.pi * (k[0] * i0 + k[1] * i1 + k[2] * i2) + ph)
+    x += 1e-3 * ffs.fill_uniform(seed + 1, x.size).reshape(sizes)
+    return x
+
+
+def fft_baseline(torch, x_dev, w, radius, steps=4):
+    """The reference's competitor: one full-grid FFT convolution per step (benchmarks/cufft-by-pytorch/
+    rfft2D.py), here on rocFFT through torch.fft, with the kernel spectrum computed once."""
+    d = x_dev.dim()
+    K = 2 * radius + 1
+    ker = torch.zeros_like(x_dev)
+    ker[(slice(0, K),) * d] = torch.from_numpy(np.asarray(w).reshape((K,) * d)).to(x_dev.device)
+    dims = tuple(range(d))
+    kh = torch.fft.rfftn(ker, dim=dims)
+    y = x_dev
+    y = torch.fft.irfftn(torch.fft.rfftn(y, dim=dims) * kh, s=x_dev.shape, dim=dims)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        y = torch.fft.irfftn(torch.fft.rfftn(y, dim=dims) * kh, s=x_dev.shape, dim=dims)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    del ker, kh, y
+    return {"value": x_dev.numel() / dt / 1e9, "unit": "GStencils/s", "ms_per_step": dt * 1e3,
+            "what": "full-grid FP64 rocFFT convolution, one rfftn/irfftn pair per step (torch.fft)"}
 
 
 def log(*a):
@@ -52,6 +93,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle sample check")
     ap.add_argument("--forms", default=None, help="comma list overriding the workload's precision forms")
+    ap.add_argument("--no-fft-baseline", action="store_true", help="skip the full-grid FFT convolution baseline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 3 plain device copies of the grid (known bytes) to calibrate HBM counters")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -63,14 +107,27 @@ def main():
     import flashfftstencil_amd as ffs
 
     torch.cuda.set_device(0)
+    if args.workload == "batch96":
+        return bench_batch(args, torch, ffs)
     wl = WORKLOADS[args.workload]
     forms = tuple(args.forms.split(",")) if args.forms else wl["forms"]
     _, radius, w, tag = ffs.named_stencil(wl["shape"], seed=wl["seed"])
     sizes, T = wl["sizes"], wl["T"]
     npts = int(np.prod(sizes))
-    x = ffs.fill_uniform(wl["seed"] + 1, npts).reshape(sizes)
+    if wl.get("init") == "sinusoids":
+        x = sinusoid_field(ffs, wl["seed"] + 1, sizes)
+    else:
+        x = ffs.fill_uniform(wl["seed"] + 1, npts).reshape(sizes)
     d_in = torch.from_numpy(x).cuda()
     stream = torch.cuda.current_stream()
+
+    if args.copy_calibration:
+        tmp = torch.empty_like(d_in)
+        for _ in range(3):
+            tmp.copy_(d_in)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 copies of {d_in.numel() * 8} bytes")
+        del tmp
 
     results = {}
     outputs = {}
@@ -164,6 +221,14 @@ def main():
             errors[f"{form}_vs_oracle_T{ts}"] = {"max_rel": rel, "rel_l2": l2}
             log(f"{form} vs oracle after {ts} steps: max-rel {rel:.3e} rel-L2 {l2:.3e}")
 
+    fftb = None
+    if len(sizes) > 1 and not args.no_fft_baseline:
+        try:
+            fftb = fft_baseline(torch, d_in, w, radius)
+            log(f"rocFFT per-step full-grid convolution: {fftb['value']:.2f} GStencil/s")
+        except RuntimeError as e:  # out of memory on the largest grids
+            log(f"fft baseline skipped: {e}")
+
     main_form = forms[0] if forms[0] in results else next(iter(results))
     r = results[main_form]
     traffic = _traffic(args.workload, main_form)
@@ -194,9 +259,84 @@ def main():
         "forms": {f: {"value": v["value"], "ms_per_step": v["ms_per_step"], "kernel_ms": v["kernel_ms"],
                       "roofline": v["roofline"], "fuse": v["info"]["fuse"]} for f, v in results.items()},
         "errors": errors,
+        "fft_baseline": fftb,
     }
     if "fp64" in results and main_form != "fp64":
         line["speedup_vs_fp64"] = r["value"] / results["fp64"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+def splitmix_ints(seed, n):
+    out, s, m = [], seed, (1 << 64) - 1
+    for _ in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & m
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def batch96_specs(ffs, seed):
+    """96 grids: sides from {512, 640, ..., 4096}, stencil from {Heat-2D, Star2D9P, Box2D9P, Box2D25P},
+    T from {16, 32, ..., 256}, weights and data sub-seeded; all draws from SplitMix64(seed)."""
+    shapes = ["Heat-2D", "Star2D9P", "Box2D9P", "Box2D25P"]
+    r = splitmix_ints(seed, 96 * 4)
+    specs = []
+    for g in range(96):
+        n0 = 512 + 128 * (r[4 * g] % 29)
+        n1 = 512 + 128 * (r[4 * g + 1] % 29)
+        shape = shapes[r[4 * g + 2] % 4]
+        T = 16 * (1 + r[4 * g + 3] % 16)
+        _, radius, w, _ = ffs.named_stencil(shape, seed=seed + 1000 + g)
+        specs.append(dict(shape=shape, sizes=(n0, n1), steps=T, radius=radius, weights=w, seed=seed + 2000 + g))
+    return specs
+
+
+def bench_batch(args, torch, ffs):
+    specs = batch96_specs(ffs, WORKLOADS["batch96"]["seed"])
+    form = (args.forms or "bf16").split(",")[0]
+    pts = sum(s["sizes"][0] * s["sizes"][1] * s["steps"] for s in specs)
+    d_in = [torch.from_numpy(ffs.fill_uniform(s["seed"], s["sizes"][0] * s["sizes"][1]).reshape(s["sizes"])).cuda()
+            for s in specs]
+    d_out = [torch.empty_like(d) for d in d_in]
+    batch = ffs.Batch(specs, precision=form)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        batch.run(d_in, d_out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        batch.run(d_in, d_out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gst = pts / (ms * 1e-3) / 1e9
+    log(f"batch96 {form}: {gst:.1f} GStencil/s, {ms:.2f} ms/run, {pts:.3e} point-steps")
+    # per-grid error against the FP64 form of the same grid
+    worst = 0.0
+    for i, s in enumerate(specs):
+        plan = ffs.Plan(2, s["sizes"], s["weights"], s["radius"], s["steps"], precision="fp64")
+        ref = torch.empty_like(d_in[i])
+        plan.run(d_in[i], ref)
+        torch.cuda.synchronize()
+        plan.close()
+        worst = max(worst, float((d_out[i] - ref).abs().max() / ref.abs().max()))
+        del ref
+    log(f"batch96: worst per-grid max-rel vs FP64 {worst:.3e}")
+    info0 = batch.info(0)
+    batch.close()
+    line = {
+        "metric": "GStencils/s (points x steps / s)", "value": gst, "unit": "GStencils/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3-split/f32-acc (f64 state)",
+        "data": "synthetic (SplitMix64 uniform[0,1) grids and weights)",
+        "config": {"workload": "batch of 96 2D grids, sides 512..4096, Heat-2D/Star2D9P/Box2D9P/Box2D25P, T 16..256",
+                   "grids": 96, "point_steps": pts, "form": form, "tile": info0["tile"]},
+        "roofline": None, "cpu_baseline": None,
+        "errors": {"worst_grid_max_rel_vs_fp64": worst},
+    }
     print(json.dumps(line), flush=True)
 
 

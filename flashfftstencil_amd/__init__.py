@@ -123,6 +123,7 @@ def load_library() -> ctypes.CDLL:
         "ffs_run_block": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p],
         "ffs_format_csv": [ctypes.c_char_p, ctypes.c_int32, P(ctypes.c_int64), ctypes.c_int64, ctypes.c_double,
                            ctypes.c_char_p, ctypes.c_int32],
+        "ffs_autotune": [P(Problem), ctypes.c_char_p, P(ctypes.c_double)],
         "ffs_batch_create": [P(Problem), ctypes.c_int32, P(ctypes.c_void_p)],
         "ffs_batch_run": [ctypes.c_void_p, P(ctypes.c_void_p), P(ctypes.c_void_p), ctypes.c_void_p],
         "ffs_batch_get_info": [ctypes.c_void_p, ctypes.c_int32, P(PlanInfo)],
@@ -258,6 +259,24 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def autotune(dims: int, sizes: Sequence[int], weights, radius: int, steps: int, precision: str = "fp64",
+             convention: str = "ref", cache: Optional[str] = None) -> Tuple[int, int, float]:
+    """Measured best (tile, fuse, ms per run) for this problem on device 0 (cached in `cache`)."""
+    lib = load_library()
+    w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).ravel())
+    p = Problem()
+    _check(lib.ffs_problem_init(ctypes.byref(p)))
+    p.dims, p.radius, p.steps = dims, radius, int(steps)
+    for i, s in enumerate(list(sizes) + [1] * (3 - len(sizes))):
+        p.sizes[i] = int(s)
+    p.weights = _dptr(w)
+    p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    p.convention = CONVENTIONS[convention] if isinstance(convention, str) else int(convention)
+    ms = ctypes.c_double()
+    _check(lib.ffs_autotune(ctypes.byref(p), cache.encode() if cache else None, ctypes.byref(ms)))
+    return p.tile, p.fuse, ms.value
 
 
 class Batch:

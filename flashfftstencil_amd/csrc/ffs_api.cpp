@@ -602,6 +602,113 @@ ffs_status ffs_run_host(ffs_plan* pl, const double* h_in, double* h_out) {
   return FFS_OK;
 }
 
+ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms) {
+  int64_t npoints = 0;
+  ffs_status st = validate(p, &npoints);
+  if (st != FFS_OK) return st;
+  char key[256];
+  std::snprintf(key, sizeof key, "d%d_r%d_p%d_c%d_n%lldx%lldx%lld_T%lld", p->dims, p->radius, p->precision,
+                p->convention, (long long)p->sizes[0], (long long)(p->dims > 1 ? p->sizes[1] : 1),
+                (long long)(p->dims > 2 ? p->sizes[2] : 1), (long long)p->steps);
+  if (cache_path) {  // a line: {"key": "...", "tile": P, "fuse": t, "ms": x}
+    if (FILE* f = std::fopen(cache_path, "r")) {
+      char line[512];
+      int tile = -1, fuse = -1;
+      double ms = 0;
+      while (std::fgets(line, sizeof line, f)) {
+        char k[256];
+        int tl, fu;
+        double m;
+        if (std::sscanf(line, " {\"key\": \"%255[^\"]\", \"tile\": %d, \"fuse\": %d, \"ms\": %lf", k, &tl, &fu, &m) == 4 &&
+            std::strcmp(k, key) == 0) {
+          tile = tl;
+          fuse = fu;
+          ms = m;
+        }
+      }
+      std::fclose(f);
+      if (tile >= 0) {
+        p->tile = tile;
+        p->fuse = fuse;
+        if (best_ms) *best_ms = ms;
+        return FFS_OK;
+      }
+    }
+  }
+  std::vector<int> tiles;
+  if (p->dims == 1) tiles = {240};
+  else if (p->dims == 3) tiles = {16};
+  else if (p->precision == FFS_PREC_MIXED_BF16 || p->precision == FFS_PREC_MIXED_BF16X2) tiles = {16, 32};
+  else tiles = {16};
+  double best = 1e300;
+  int best_tile = 0, best_fuse = 0;
+  double *din = nullptr, *dout = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&]() {
+    if (din) (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  for (int tile : tiles) {
+    const int vmin = p->dims == 1 ? 16 : 2;
+    const int64_t tmax = p->radius == 0 ? 1 : std::max<int64_t>(1, (tile - vmin) / (2 * p->radius));
+    for (int64_t t = 1; t <= std::min<int64_t>(tmax, std::max<int64_t>(p->steps, 1)); t++) {
+      if (t > 8 && (t & (t - 1)) != 0 && t != tmax) continue;  // all depths up to 8, then powers of two + max
+      ffs_problem c = *p;
+      c.tile = tile;
+      c.fuse = (int32_t)t;
+      ffs_plan* pl = nullptr;
+      if (ffs_plan_create(&c, &pl) != FFS_OK) continue;
+      if (!din) {
+        hipError_t e = hipMalloc(&din, sizeof(double) * npoints);
+        if (e == hipSuccess) e = hipMalloc(&dout, sizeof(double) * npoints);
+        if (e == hipSuccess) e = hipMemset(din, 0, sizeof(double) * npoints);
+        if (e == hipSuccess) e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e != hipSuccess) {
+          ffs_plan_destroy(pl);
+          cleanup();
+          return fail(FFS_ERR_OUT_OF_MEMORY, "autotune scratch: %s", hipGetErrorString(e));
+        }
+      }
+      double ms_best = 1e300;
+      for (int rep = 0; rep < 3; rep++) {  // rep 0 captures the graph and warms up
+        (void)hipEventRecord(e0, nullptr);
+        st = ffs_run(pl, din, dout, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (st != FFS_OK) break;
+        if (rep > 0) ms_best = std::min(ms_best, (double)ms);
+      }
+      ffs_plan_destroy(pl);
+      if (st != FFS_OK) {
+        cleanup();
+        return st;
+      }
+      if (ms_best < best) {
+        best = ms_best;
+        best_tile = tile;
+        best_fuse = (int)t;
+      }
+    }
+  }
+  cleanup();
+  if (best_fuse == 0) return fail(FFS_ERR_UNSUPPORTED, "no admissible configuration");
+  p->tile = best_tile;
+  p->fuse = best_fuse;
+  if (best_ms) *best_ms = best;
+  if (cache_path) {
+    if (FILE* f = std::fopen(cache_path, "a")) {
+      std::fprintf(f, "{\"key\": \"%s\", \"tile\": %d, \"fuse\": %d, \"ms\": %.6f}\n", key, best_tile, best_fuse, best);
+      std::fclose(f);
+    }
+  }
+  return FFS_OK;
+}
+
 ffs_status ffs_batch_create(const ffs_problem* problems, int32_t count, ffs_batch** out) {
   if (!out) return fail(FFS_ERR_INVALID_ARG, "out is NULL");
   *out = nullptr;

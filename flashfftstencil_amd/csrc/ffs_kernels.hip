@@ -211,6 +211,11 @@ __device__ __forceinline__ void store_pair(const TileArgs& a, const PairCoord& p
 
 template <int KIND>
 __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
+  // The spectrum (256 complex, 4 KB) is staged in LDS once per workgroup and read per tile pair,
+  // which keeps the kernel within 128 VGPRs (4 waves per SIMD).
+  __shared__ double2 sH[256];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) sH[i] = double2{a.H[i], a.H[256 + i]};
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
@@ -221,12 +226,6 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
   load_ops(f0, a.ops, 0, lane);
   if (KIND != KIND_2D16) load_ops(f1, a.ops, 1, lane);
   const OpsAxis& g1 = (KIND == KIND_2D16) ? f0 : f1;
-  d4 hr, hi;
-#pragma unroll
-  for (int v = 0; v < 4; v++) {
-    hr[v] = a.H[v * kLanes + lane];
-    hi[v] = a.H[256 + v * kLanes + lane];
-  }
 
   int64_t tp = gw;
   if (tp >= a.pairs) return;
@@ -249,10 +248,12 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
     to_chunks(sr, cr);
     to_chunks(si, ci);
     cstage<false>(cr, ci, g1, sr, si);  // contract axis 1 -> Y[k0 = q+4v][k1 = j]
-    {
-      const d4 yr = sr, yi = si;
-      sr = yr * hr - yi * hi;
-      si = yr * hi + yi * hr;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const double2 h = sH[v * kLanes + lane];
+      const double yr = sr[v], yi = si[v];
+      sr[v] = yr * h.x - yi * h.y;
+      si[v] = yr * h.y + yi * h.x;
     }
     to_chunks(sr, cr);
     to_chunks(si, ci);

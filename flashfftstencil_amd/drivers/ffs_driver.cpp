@@ -52,7 +52,8 @@ static void print_help() {
       "  --convention C       ref (default, out[p] = sum_a w[a] in[p-a]) | centred (out[p] = sum_o w[o+r] in[p+o])\n"
       "  --seed S             seed of the weights and the initial grid (default 0x5EED0000)\n"
       "  --fuse t             fused time depth (default automatic)\n"
-      "  --no-graph           plain launches instead of a hipGraph replay\n",
+      "  --no-graph           plain launches instead of a hipGraph replay\n"
+      "  --tune [CACHE]       time the admissible tile/depth choices first (cached in CACHE, a JSON-lines file)\n",
       kDims, kDims, shapes, kDims, kDims);
 }
 
@@ -78,6 +79,8 @@ int main(int argc, char** argv) {
   std::string custom, prec = "fp64", conv = "ref";
   unsigned long long seed = 0x5EED0000ULL;
   int fuse = 0, use_graph = 1;
+  bool tune = false;
+  std::string tune_cache;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
     auto need = [&](const char* what) -> const char* {
@@ -103,6 +106,9 @@ int main(int argc, char** argv) {
       fuse = std::atoi(need("--fuse"));
     } else if (a == "--no-graph") {
       use_graph = 0;
+    } else if (a == "--tune") {
+      tune = true;
+      if (i + 1 < argc && argv[i + 1][0] != '-' && std::strchr(argv[i + 1], '.') != nullptr) tune_cache = argv[++i];
     } else if (a.size() > 2 && a[0] == '-' && a[1] == '-') {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       print_help();
@@ -180,6 +186,7 @@ int main(int argc, char** argv) {
   p.fuse = fuse;
   p.use_graph = use_graph;
 
+  if (tune) CHECK_FFS(ffs_autotune(&p, tune_cache.empty() ? nullptr : tune_cache.c_str(), nullptr));
   ffs_plan* plan = nullptr;
   CHECK_FFS(ffs_plan_create(&p, &plan));
   long long npts = 1;

@@ -130,6 +130,13 @@ ffs_status ffs_run_host(ffs_plan* plan, const double* h_in, double* h_out);
  * d_out = S^t(d_in).  One kernel launch; used to time the tile kernel alone. */
 ffs_status ffs_run_block(ffs_plan* plan, const double* d_in, double* d_out, int32_t tail, void* stream);
 
+/* Autotuner: time the admissible (tile, fused depth) plans of `p` on its device (full T-step runs on a
+ * scratch grid of p's sizes) and write the fastest into p->tile and p->fuse.  With a non-NULL
+ * `cache_path`, results are read from / appended to that JSON-lines file keyed by (dims, radius,
+ * precision, convention, sizes, T), so each configuration is measured once.  best_ms (may be NULL)
+ * receives the winner's time per run.  No reference counterpart (the reference hard-codes its tile). */
+ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms);
+
 /* Batches of independent 2D grids (different sizes, stencils and T).  Block b of every grid that still
  * has blocks runs in ONE persistent launch over the concatenated tile-pair list of those grids; each
  * grid keeps its own fused depth, tail and spectrum.  All problems must be 2D and share one BF16 mixed
