@@ -34,7 +34,7 @@ WORKLOADS = {
     "box2d25p": dict(shape="Box2D25P", sizes=(8192, 8192), T=256, seed=0x5EED0002, forms=("bf16", "fp64")),
     "box2d9p": dict(shape="Box2D9P", sizes=(8192, 8192), T=128, seed=0x5EED0001, forms=("fp64",)),
     "heat1d": dict(shape="Heat-1D", sizes=(1048576,), T=64, seed=0x5EED0000, forms=("fp64",)),
-    "box3d27p": dict(shape="Box3D27P", sizes=(512, 512, 512), T=32, seed=0x5EED0003, forms=("f32", "bf16", "fp64"),
+    "box3d27p": dict(shape="Box3D27P", sizes=(512, 512, 512), T=32, seed=0x5EED0003, forms=("bf16", "f32", "fp64"),
                      init="sinusoids"),
     "batch96": dict(seed=0x5EED0004, forms=("bf16",)),
 }

@@ -44,6 +44,22 @@ __device__ __forceinline__ int64_t wrap_near(int64_t i, int64_t n) {
   return wrap64(i, n);
 }
 
+// Complex product (yr + i yi)(hr + i hi) with the rounding pinned (explicit FMAs), so every kernel
+// that applies the spectrum produces bitwise the same result for the same inputs.
+template <typename T>
+__device__ __forceinline__ void cmul(T& yr, T& yi, T hr, T hi) {
+  const T r = __builtin_fma(yr, hr, -(yi * hi));
+  const T i = __builtin_fma(yr, hi, yi * hr);
+  yr = r;
+  yi = i;
+}
+__device__ __forceinline__ void cmul(float& yr, float& yi, float hr, float hi) {
+  const float r = __builtin_fmaf(yr, hr, -(yi * hi));
+  const float i = __builtin_fmaf(yr, hi, yi * hr);
+  yr = r;
+  yi = i;
+}
+
 // XCD-aware workgroup order: the hardware hands workgroup b to XCD (b mod 8); renumber so that each
 // XCD receives one contiguous range of logical workgroups (neighbouring tiles share its L2).
 __device__ __forceinline__ int64_t xcd_logical_block(int64_t b, int64_t G) {
@@ -251,9 +267,10 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const double2 h = sH[v * kLanes + lane];
-      const double yr = sr[v], yi = si[v];
-      sr[v] = yr * h.x - yi * h.y;
-      si[v] = yr * h.y + yi * h.x;
+      double yr = sr[v], yi = si[v];
+      cmul(yr, yi, h.x, h.y);
+      sr[v] = yr;
+      si[v] = yi;
     }
     to_chunks(sr, cr);
     to_chunks(si, ci);
@@ -291,7 +308,7 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
 // The FP32 form uses one v_mfma_f32_16x16x4f32 per 4-row chunk, with
 // the DFT tables stored in the permuted chunk order (rows 4k + c) that the FP32 accumulator implies.
 //
-// Every tile window is taken relative to a DC offset c (FP64, one sample of the window): the FFT path
+// Every tile window is taken relative to a DC offset c (FP64, the window's center sample): the FFT path
 // runs on x - c and the store adds c * K(0)^t back in FP64 (a constant field is an eigenvector with
 // eigenvalue K(0)).  This keeps the split/rounding errors relative to the fluctuation, not to the
 // (large, slowly varying) level of the field, and is what holds the mixed forms below 1e-6 over
@@ -490,21 +507,23 @@ __device__ __forceinline__ void store_pair_m(const TileArgs& a, const PairCoord&
   }
 }
 
-// Wave-uniform copy of lane 0's value.
-__device__ __forceinline__ double lane0(double v) {
+// Wave-uniform copy of lane L's value.
+__device__ __forceinline__ double lane_value(double v, int L) {
   const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b), hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, L), hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), L);
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// One tile pair through the mixed pipeline.  The DC offset of each window is its first sample (row 0,
-// column 0: lane 0, element 0), which tracks the local mean once the field is smooth and bounds the
-// fluctuation by the window's range before.
+// One tile pair through the mixed pipeline.  The DC offset of each window is its center sample, which
+// tracks the local mean of a smooth field and bounds the fluctuation by half the window's range
+// otherwise.  2D: window (8, 8) = lane 8 + 16*2, element 0; 1D: window sample 120 = (row 0, col 8) =
+// lane 8, element 0.
 template <int KIND, int NTERMS, typename Ops>
 __device__ __forceinline__ void mixed_pair(const TileArgs& a, const PairCoord& pc, int q, int j, const double (&xr)[4],
                                            const double (&xi)[4], const Ops& f0, const Ops& g1, const f4& hr,
                                            const f4& hi) {
-  const double cA = lane0(xr[0]), cB = lane0(xi[0]);
+  const int cl = KIND == KIND_2D16 ? 40 : 8;
+  const double cA = lane_value(xr[0], cl), cB = lane_value(xi[0], cl);
   f4 ar, ai;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
@@ -514,8 +533,13 @@ __device__ __forceinline__ void mixed_pair(const TileArgs& a, const PairCoord& p
   f4 sr, si;
   mstage<false, NTERMS>(ar, ai, f0, sr, si);
   mstage<false, NTERMS>(sr, si, g1, ar, ai);  // Y[k0 = 4q+v][k1 = j]
-  sr = ar * hr - ai * hi;
-  si = ar * hi + ai * hr;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    float yr = ar[v], yi = ai[v];
+    cmul(yr, yi, hr[v], hi[v]);
+    sr[v] = yr;
+    si[v] = yi;
+  }
   mstage<true, NTERMS>(sr, si, f0, ar, ai);
   mstage<true, NTERMS>(ar, ai, g1, sr, si);  // X'[n0 = 4q+v][n1 = j]
   store_pair_m<KIND>(a, pc, q, j, sr, si, cA * a.dc_gain, cB * a.dc_gain);
@@ -708,7 +732,8 @@ __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const
   const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
   double xa[16], xb[16];
   gather32(a, pc, q, j, xa, xb);
-  const double cA = lane0(xa[0]), cB = lane0(xb[0]);
+  // DC offset: window center (16, 16) = block 3, lane 0, element 0
+  const double cA = lane_value(xa[12], 0), cB = lane_value(xb[12], 0);
   Blk4 zr, zi, orr, oi;
 #pragma unroll
   for (int b = 0; b < 4; b++)
@@ -726,9 +751,10 @@ __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const
       const int e = (b * 4 + v) * kLanes + lane;
       const float hr = H_FROM_LDS ? sH[e][0] : (float)a.H[e];
       const float hi = H_FROM_LDS ? sH[e][1] : (float)a.H[1024 + e];
-      const float yr = zr.b[b][v], yi = zi.b[b][v];
-      zr.b[b][v] = yr * hr - yi * hi;
-      zi.b[b][v] = yr * hi + yi * hr;
+      float yr = zr.b[b][v], yi = zi.b[b][v];
+      cmul(yr, yi, hr, hi);
+      zr.b[b][v] = yr;
+      zi.b[b][v] = yi;
     }
   stage32<true, NTERMS>(zr, zi, sF, lane, orr, oi);
   stage32<true, NTERMS>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
@@ -888,12 +914,12 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
     const int64_t w0 = o0 - a.h0, w1 = o1 - a.h1, w2A = o2A - a.h2, w2B = o2B - a.h2;
     const bool interior = w0 >= 0 && w0 + 16 <= a.n0 && w1 >= 0 && w1 + 16 <= a.n1 && w2A >= 0 &&
                           (hasB ? w2B : w2A) + 16 <= a.n2;
-    // DC offsets (mixed forms): the window's first sample, a wave-uniform load
+    // DC offsets (mixed forms): the window's center sample (8, 8, 8), a wave-uniform load
     double cA = 0.0, cB = 0.0;
     if (mixed) {
-      const int64_t base = (wrap_near(w0, a.n0) * a.n1 + wrap_near(w1, a.n1)) * a.n2;
-      cA = a.src[base + wrap_near(w2A, a.n2)];
-      cB = hasB ? a.src[base + wrap_near(w2B, a.n2)] : 0.0;
+      const int64_t base = (wrap_near(w0 + 8, a.n0) * a.n1 + wrap_near(w1 + 8, a.n1)) * a.n2;
+      cA = a.src[base + wrap_near(w2A + 8, a.n2)];
+      cB = hasB ? a.src[base + wrap_near(w2B + 8, a.n2)] : 0.0;
     }
     // ---- fill: thread (y = tid / 16, z = tid % 16) loads column z of row y in every slice x ----
     {
@@ -960,9 +986,10 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const V hr = (V)H[v * kLanes], hi = (V)H[4096 + v * kLanes];
-        const V yr = br[v], yi = bi[v];
-        br[v] = yr * hr - yi * hi;
-        bi[v] = yr * hi + yi * hr;
+        V yr = br[v], yi = bi[v];
+        cmul(yr, yi, hr, hi);
+        br[v] = yr;
+        bi[v] = yi;
       }
       V4 cr, ci;
       P::template stage<true>(br, bi, f, cr, ci);  // [k0 = row][n1 = lane]
