@@ -400,17 +400,20 @@ __device__ __forceinline__ void mstage(const f4& ar, const f4& ai, const OpsBF16
   bf8 R[NTERMS], I[NTERMS];
   split_a<NTERMS>(ar, R);
   split_a<NTERMS>(ai, I);
+  // Small split products first, the two dominant ones (pairing 0: ah*Fh) last: the FP32 accumulator
+  // then rounds at full magnitude twice per output instead of 2*NTERMS times.
   f4 re = {0.f, 0.f, 0.f, 0.f}, im = re;
 #pragma unroll
-  for (int g = 0; g < NTERMS; g++) {
+  for (int g = NTERMS - 1; g >= 1; g--) {
     re = FFS_MFMA_BF16(R[g], f.r[g], re);
     im = FFS_MFMA_BF16(R[g], INV ? f.ni[g] : f.i[g], im);
-  }
-#pragma unroll
-  for (int g = 0; g < NTERMS; g++) {
     re = FFS_MFMA_BF16(I[g], INV ? f.i[g] : f.ni[g], re);
     im = FFS_MFMA_BF16(I[g], f.r[g], im);
   }
+  re = FFS_MFMA_BF16(R[0], f.r[0], re);
+  im = FFS_MFMA_BF16(R[0], INV ? f.ni[0] : f.i[0], im);
+  re = FFS_MFMA_BF16(I[0], INV ? f.i[0] : f.ni[0], re);
+  im = FFS_MFMA_BF16(I[0], f.r[0], im);
   outr = re;
   outi = im;
 }
@@ -622,17 +625,17 @@ __device__ __forceinline__ void split8(const f4& x0, const f4& x1, bf8& h, bf8& 
   }
 }
 
-// acc += (ah + am + al) x (Fh + Fm + Fl), products of order <= 2^-16 only.
+// acc += (ah + am + al) x (Fh + Fm + Fl) without the dominant product ah x Fh, i.e. the split products
+// of order 2^-8 .. 2^-16 (added first, while the accumulator is still small).
 template <int NTERMS>
-__device__ __forceinline__ f4 prod32(const bf8& ah, const bf8& am, const bf8& al, const bf8 (&F)[3], f4 acc) {
-  acc = FFS_MFMA_BF16(ah, F[0], acc);
-  acc = FFS_MFMA_BF16(ah, F[1], acc);
-  acc = FFS_MFMA_BF16(am, F[0], acc);
-  acc = FFS_MFMA_BF16(ah, F[2], acc);
+__device__ __forceinline__ f4 prod32_small(const bf8& ah, const bf8& am, const bf8& al, const bf8 (&F)[3], f4 acc) {
   if (NTERMS == 3) {
-    acc = FFS_MFMA_BF16(am, F[1], acc);
     acc = FFS_MFMA_BF16(al, F[0], acc);
+    acc = FFS_MFMA_BF16(am, F[1], acc);
   }
+  acc = FFS_MFMA_BF16(ah, F[2], acc);
+  acc = FFS_MFMA_BF16(am, F[0], acc);
+  acc = FFS_MFMA_BF16(ah, F[1], acc);
   return acc;
 }
 
@@ -657,10 +660,15 @@ __device__ __forceinline__ void stage32(const Blk4& zr, const Blk4& zi, const u4
       load_f32ops(sF, 2, Kb, lane, Fn);
       f4 re = {0.f, 0.f, 0.f, 0.f}, iv = re;
       // forward: Re = zr Fr + zi (-Fi), Im = zr Fi + zi Fr;  inverse: Re = zr Fr + zi Fi, Im = zr (-Fi) + zi Fr
-      re = prod32<NTERMS>(rh, rm, rl, Fr, re);
-      iv = prod32<NTERMS>(rh, rm, rl, INV ? Fn : Fi, iv);
-      re = prod32<NTERMS>(ih, im, il, INV ? Fi : Fn, re);
-      iv = prod32<NTERMS>(ih, im, il, Fr, iv);
+      // small split products first, the two dominant ones last (two full-magnitude roundings per output)
+      re = prod32_small<NTERMS>(rh, rm, rl, Fr, re);
+      iv = prod32_small<NTERMS>(rh, rm, rl, INV ? Fn : Fi, iv);
+      re = prod32_small<NTERMS>(ih, im, il, INV ? Fi : Fn, re);
+      iv = prod32_small<NTERMS>(ih, im, il, Fr, iv);
+      re = FFS_MFMA_BF16(rh, Fr[0], re);
+      iv = FFS_MFMA_BF16(rh, INV ? Fn[0] : Fi[0], iv);
+      re = FFS_MFMA_BF16(ih, INV ? Fi[0] : Fn[0], re);
+      iv = FFS_MFMA_BF16(ih, Fr[0], iv);
       outr.b[2 * I + Kb] = re;
       outi.b[2 * I + Kb] = iv;
     }
@@ -940,8 +948,8 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
       }
     }
     __syncthreads();
-    // ---- phase 1: slices a = 4w + s; contract axis 2, then axis 1 ----
-#pragma unroll 1
+    // ---- phase 1: slices a = 4w + s; contract axis 2, then axis 1 (two slices in flight) ----
+#pragma unroll 2
     for (int s = 0; s < 4; s++) {
       const int sl = 4 * w + s;
       V4 ar, ai;
@@ -976,7 +984,7 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
       }
     }
     __syncthreads();
-#pragma unroll 1
+#pragma unroll 2
     for (int s = 0; s < 4; s++) {
       const int k2 = 4 * w + s;
       V4 br, bi;
@@ -1003,8 +1011,8 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
       }
     }
     __syncthreads();
-    // ---- phase 3: slices n0 = 4w + s; inverse axis 2; store ----
-#pragma unroll 1
+    // ---- phase 3: slices n0 = 4w + s; inverse axis 2; store (two slices in flight) ----
+#pragma unroll 2
     for (int s = 0; s < 4; s++) {
       const int n0 = 4 * w + s;
       V4 ar, ai;
