@@ -794,11 +794,14 @@ __global__ __launch_bounds__(256, 2) void k_tile32_bf16(TileArgs a) {
 }
 
 // Multi-grid launch: one fused block of every grid still running, as one persistent launch over the
-// concatenated work list.  items[g] describes grid g's block (source, destination, depth, spectrum);
+// concatenated work list.  Each item carries its own spectrum, read per tile pair from its FP64 table
+// (L2-resident); with only the DFT operands in LDS this kernel stays near 160 VGPRs and runs 3 waves
+// per SIMD (measured faster for batches than 2 waves).  The single-grid kernel above stages its one
+// spectrum in LDS instead (measured faster for one large grid).  items[g] describes grid g's block (source, destination, depth, spectrum);
 // prefix[g] is the first work index of grid g (prefix[n] = total).  All items cost the same (one 32x32
 // tile pair), so waves take them in a fixed strided order: balanced and bitwise reproducible.
 template <int NTERMS>
-__global__ __launch_bounds__(256, 2) void k_batch32_bf16(const TileArgs* __restrict__ items,
+__global__ __launch_bounds__(256, 3) void k_batch32_bf16(const TileArgs* __restrict__ items,
                                                          const int64_t* __restrict__ prefix, int32_t n) {
   __shared__ u4 sF[3 * 2 * 3 * kLanes];
   stage_ops32(sF, items[0].opsm);
@@ -821,7 +824,7 @@ __global__ __launch_bounds__(256, 2) void k_batch32_bf16(const TileArgs* __restr
 
 hipError_t launch_batch(const TileArgs* d_items, const int64_t* d_prefix, int32_t n, int32_t precision, hipStream_t s,
                         int num_cus) {
-  const dim3 g((unsigned)num_cus * 2), b(256);
+  const dim3 g((unsigned)num_cus * 3), b(256);
   if (precision == PREC_BF16)
     hipLaunchKernelGGL(k_batch32_bf16<3>, g, b, 0, s, d_items, d_prefix, n);
   else if (precision == PREC_BF16X2)
