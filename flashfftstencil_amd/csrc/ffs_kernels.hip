@@ -752,6 +752,9 @@ __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const
     }
   stage32<false, NTERMS>(zr, zi, sF, lane, orr, oi);
   stage32<false, NTERMS>(orr, oi, sF, lane, zr, zi);  // Y[k0][k1], block order
+  // Keep the spectrum reads here: hoisted to the top of the pair they would hold 32 VGPRs through two
+  // stages and cost the third wave per SIMD.
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int b = 0; b < 4; b++)
 #pragma unroll
@@ -780,7 +783,7 @@ __device__ __forceinline__ void stage_ops32(u4* sF, const uint32_t* __restrict__
 }
 
 template <int NTERMS>
-__global__ __launch_bounds__(256, 2) void k_tile32_bf16(TileArgs a) {
+__global__ __launch_bounds__(256, 3) void k_tile32_bf16(TileArgs a) {
   __shared__ u4 sF[3 * 2 * 3 * kLanes];  // B operands, 18 KB
   __shared__ f2 sH[4 * 4 * kLanes];      // spectrum [block][reg][lane] (re, im), 8 KB
   stage_ops32(sF, a.opsm);
@@ -1282,8 +1285,8 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
   const int64_t waves_needed = a.pairs;
   int64_t blocks = (waves_needed + 3) / 4;
   // 8 workgroups (32 waves) per CU at most, grid-stride beyond; the 32-point kernel stages its tables
-  // in LDS per workgroup and runs 2 waves per SIMD, so it launches exactly the resident workgroups.
-  const int64_t cap = (int64_t)num_cus * (kind == KIND_2D32 ? 2 : 8);
+  // in LDS per workgroup and runs 3 waves per SIMD, so it launches exactly the resident workgroups.
+  const int64_t cap = (int64_t)num_cus * (kind == KIND_2D32 ? 3 : 8);
   if (blocks > cap) blocks = cap;
   const dim3 g((unsigned)blocks), b(256);
   if (kind == KIND_2D16) {
