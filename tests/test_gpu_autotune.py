@@ -1,0 +1,29 @@
+"""Autotuner on the GPU: it measures the admissible (tile, depth) plans, returns one of them, caches the
+winner (a second call reads the cache), and the tuned plan is as exact as any other."""
+import json
+import os
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape,size,steps,form", [("Box2D9P", (300, 260), 12, "fp64"),
+                                                   ("Box2D25P", (256, 256), 9, "bf16"),
+                                                   ("Heat-1D", (5000,), 40, "fp64")])
+def test_autotune_picks_admissible_and_caches(ffs, oracle, torch_cuda, tmp_path, shape, size, steps, form):
+    dims, r, w, _ = ffs.named_stencil(shape, seed=4)
+    cache = str(tmp_path / "tune.jsonl")
+    tile, fuse, ms = ffs.autotune(dims, size, w, r, steps, precision=form, cache=cache)
+    assert ms > 0 and fuse >= 1
+    assert tile in ((240,) if dims == 1 else ((16, 32) if form == "bf16" else (16,)))
+    lines = [json.loads(l) for l in open(cache)]
+    assert len(lines) == 1 and lines[0]["tile"] == tile and lines[0]["fuse"] == fuse
+    assert ffs.autotune(dims, size, w, r, steps, precision=form, cache=cache)[:2] == (tile, fuse)
+    assert len(open(cache).readlines()) == 1  # served from the cache
+    x = oracle.uniform(9, size)
+    plan = ffs.Plan(dims, size, w, r, steps, precision=form, tile=tile, fuse=fuse)
+    got = plan.run_host(x)
+    plan.close()
+    tol = 1e-12 if form == "fp64" else 1e-6
+    assert oracle.errors(got, oracle.sweep(x, r, w, steps))[1] <= tol
