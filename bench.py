@@ -79,6 +79,35 @@ def fft_baseline(torch, x_dev, w, radius, steps=4):
             "what": "full-grid FP64 rocFFT convolution, one rfftn/irfftn pair per step (torch.fft)"}
 
 
+def conv_step(torch, x, w, radius):
+    """One periodic stencil step as a library convolution: circular padding of 2r cells on the low side
+    of every axis, then torch's FP64 cross-correlation with the flipped weights, which is the reference
+    convention out[p] = sum_a w[a] in[p - a] (src/2D/check_correct.hpp:64-100)."""
+    d = x.dim()
+    K = 2 * radius + 1
+    wt = torch.as_tensor(np.asarray(w, dtype=np.float64).reshape((K,) * d)[(slice(None, None, -1),) * d].copy(),
+                         device=x.device).reshape((1, 1) + (K,) * d)
+    conv = {1: torch.nn.functional.conv1d, 2: torch.nn.functional.conv2d, 3: torch.nn.functional.conv3d}[d]
+    pad = (2 * radius, 0) * d
+    return lambda y: conv(torch.nn.functional.pad(y[None, None], pad, mode="circular"), wt)[0, 0]
+
+
+def conv_baseline(torch, x_dev, w, radius, steps=3):
+    """The reference's other competitor (benchmarks/cudnn/cudnn-test.cpp: a library FP64 convolution per
+    step), here torch's FP64 convolution on ROCm with circular padding, one call per step."""
+    step = conv_step(torch, x_dev, w, radius)
+    y = step(x_dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        y = step(y)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    del y
+    return {"value": x_dev.numel() / dt / 1e9, "unit": "GStencils/s", "ms_per_step": dt * 1e3,
+            "what": "FP64 torch.nn.functional.conv%dd with circular padding, one call per step" % x_dev.dim()}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -93,7 +122,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle sample check")
     ap.add_argument("--forms", default=None, help="comma list overriding the workload's precision forms")
-    ap.add_argument("--no-fft-baseline", action="store_true", help="skip the full-grid FFT convolution baseline")
+    ap.add_argument("--no-fft-baseline", action="store_true", help="skip the rocFFT and library-convolution baselines")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 plain device copies of the grid (known bytes) to calibrate HBM counters")
     args = ap.parse_args()
@@ -229,6 +258,14 @@ def main():
         except RuntimeError as e:  # out of memory on the largest grids
             log(f"fft baseline skipped: {e}")
 
+    convb = None
+    if not args.no_fft_baseline:
+        try:
+            convb = conv_baseline(torch, d_in, w, radius)
+            log(f"library convolution per step: {convb['value']:.2f} GStencil/s")
+        except RuntimeError as e:  # out of memory on the largest grids
+            log(f"convolution baseline skipped: {e}")
+
     main_form = forms[0] if forms[0] in results else next(iter(results))
     r = results[main_form]
     traffic = _traffic(args.workload, main_form)
@@ -260,6 +297,7 @@ def main():
                       "roofline": v["roofline"], "fuse": v["info"]["fuse"]} for f, v in results.items()},
         "errors": errors,
         "fft_baseline": fftb,
+        "conv_baseline": convb,
     }
     if "fp64" in results and main_form != "fp64":
         line["speedup_vs_fp64"] = r["value"] / results["fp64"]["value"]
