@@ -27,15 +27,16 @@ PEAKS = {  # MI355X published dense peaks
     "bf16": ("mfma", 2.5e15, "TFLOP/s"),
     "f32": ("mfma", 157.3e12, "TFLOP/s"),
     "bf16x2": ("mfma", 2.5e15, "TFLOP/s"),
+    "direct": ("valu", 78.6e12, "TFLOP/s"),  # FP64 vector peak
 }
 HBM_PEAK = 8.0e12
 
 WORKLOADS = {
-    "box2d25p": dict(shape="Box2D25P", sizes=(8192, 8192), T=256, seed=0x5EED0002, forms=("bf16", "fp64")),
-    "box2d9p": dict(shape="Box2D9P", sizes=(8192, 8192), T=128, seed=0x5EED0001, forms=("fp64",)),
-    "heat1d": dict(shape="Heat-1D", sizes=(1048576,), T=64, seed=0x5EED0000, forms=("fp64",)),
-    "box3d27p": dict(shape="Box3D27P", sizes=(512, 512, 512), T=32, seed=0x5EED0003, forms=("bf16", "f32", "fp64"),
-                     init="sinusoids"),
+    "box2d25p": dict(shape="Box2D25P", sizes=(8192, 8192), T=256, seed=0x5EED0002, forms=("bf16", "fp64", "direct")),
+    "box2d9p": dict(shape="Box2D9P", sizes=(8192, 8192), T=128, seed=0x5EED0001, forms=("fp64", "direct")),
+    "heat1d": dict(shape="Heat-1D", sizes=(1048576,), T=64, seed=0x5EED0000, forms=("fp64", "direct")),
+    "box3d27p": dict(shape="Box3D27P", sizes=(512, 512, 512), T=32, seed=0x5EED0003,
+                     forms=("bf16", "f32", "fp64", "direct"), init="sinusoids"),
     "batch96": dict(seed=0x5EED0004, forms=("bf16",)),
 }
 
@@ -77,6 +78,11 @@ def fft_baseline(torch, x_dev, w, radius, steps=4):
     del ker, kh, y
     return {"value": x_dev.numel() / dt / 1e9, "unit": "GStencils/s", "ms_per_step": dt * 1e3,
             "what": "full-grid FP64 rocFFT convolution, one rfftn/irfftn pair per step (torch.fft)"}
+
+
+def form_args(form):
+    """Plan arguments of a bench form: a precision of the FFT path, or "direct" (FP64 stencil sum)."""
+    return dict(precision="fp64", algorithm="direct") if form == "direct" else dict(precision=form)
 
 
 def conv_step(torch, x, w, radius):
@@ -162,7 +168,7 @@ def main():
     outputs = {}
     for form in forms:
         try:
-            plan = ffs.Plan(len(sizes), sizes, w, radius, T, precision=form)
+            plan = ffs.Plan(len(sizes), sizes, w, radius, T, **form_args(form))
         except ffs.FFSError as e:
             log(f"form {form} unavailable: {e}")
             continue
@@ -203,7 +209,7 @@ def main():
         )
         outputs[form] = d_out
         log(f"{form}: {gst:.1f} GStencil/s, {ms:.2f} ms/run, block kernel {kms:.3f} ms, "
-            f"{achieved / 1e12:.2f} TFLOP/s ({achieved / peak:.1%} of {form} MFMA peak), P={info['tile']} "
+            f"{achieved / 1e12:.2f} TFLOP/s ({achieved / peak:.1%} of {form} {bound} peak), P={info['tile']} "
             f"t={info['fuse']} tail={info['fuse_tail']} V={info['valid']} blocks={info['blocks']}")
         plan.close()
 
@@ -240,7 +246,7 @@ def main():
                          f"sweep (oracle/oracle.c, OpenMP {threads} threads), {cpu_s:.1f} s"}
         log(f"cpu baseline: {cpu['value']:.3f} GStencil/s on {threads} threads ({ts} steps, {cpu_s:.1f} s)")
         for form in outputs:
-            plan = ffs.Plan(len(sizes), sizes, w, radius, ts, precision=form)
+            plan = ffs.Plan(len(sizes), sizes, w, radius, ts, **form_args(form))
             d = torch.empty_like(d_in)
             plan.run(d_in, d)
             torch.cuda.synchronize()
@@ -283,7 +289,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "bf16": "bf16x3-split/f32-acc (f64 state)", "bf16x2": "bf16x2-split/f32-acc (f64 state)",
-                  "f32": "f32 (f64 state)"}[main_form],
+                  "f32": "f32 (f64 state)", "direct": "f64 (direct sum)"}[main_form],
         "data": "synthetic (SplitMix64 uniform[0,1) grid and weights)",
         "config": {
             "workload": f"{wl['shape']} {'x'.join(map(str, sizes))} T={T}",

@@ -36,6 +36,7 @@ STATUS_NAMES = {
 }
 PRECISIONS = {"fp64": 0, "bf16": 1, "mixed": 1, "f32": 2, "bf16x2": 3}
 CONVENTIONS = {"ref": 0, "centred": 1, "centered": 1}
+ALGORITHMS = {"fft": 0, "direct": 1, "auto": 2}
 
 
 class FFSError(RuntimeError):
@@ -57,7 +58,8 @@ class Problem(ctypes.Structure):
         ("fuse", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
         ("device", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("algorithm", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -78,6 +80,8 @@ class PlanInfo(ctypes.Structure):
         ("launches", ctypes.c_int64),
         ("flops_per_block", ctypes.c_double),
         ("bytes_per_block", ctypes.c_double),
+        ("algorithm", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
     def as_dict(self) -> dict:
@@ -194,7 +198,7 @@ class Plan:
 
     def __init__(self, dims: int, sizes: Sequence[int], weights, radius: int, steps: int,
                  precision: str = "fp64", convention: str = "ref", tile: int = 0, fuse: int = 0,
-                 use_graph: bool = True, device: int = 0):
+                 use_graph: bool = True, device: int = 0, algorithm: str = "fft"):
         lib = load_library()
         self._w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).ravel())
         p = Problem()
@@ -212,6 +216,7 @@ class Plan:
         p.fuse = fuse
         p.use_graph = 1 if use_graph else 0
         p.device = device
+        p.algorithm = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
         self.problem = p
         self.sizes = tuple(sz[:dims])
         self.dims = dims
@@ -262,8 +267,17 @@ class Plan:
 
 
 def autotune(dims: int, sizes: Sequence[int], weights, radius: int, steps: int, precision: str = "fp64",
-             convention: str = "ref", cache: Optional[str] = None) -> Tuple[int, int, float]:
-    """Measured best (tile, fuse, ms per run) for this problem on device 0 (cached in `cache`)."""
+             convention: str = "ref", cache: Optional[str] = None, algorithm: str = "fft") -> Tuple[int, int, float]:
+    """Measured best (tile, fuse, ms per run) for this problem on device 0 (cached in `cache`).  With
+    algorithm "auto" the winner's algorithm is in :func:`autotune_full`."""
+    a, tile, fuse, ms = autotune_full(dims, sizes, weights, radius, steps, precision, convention, cache, algorithm)
+    return tile, fuse, ms
+
+
+def autotune_full(dims: int, sizes: Sequence[int], weights, radius: int, steps: int, precision: str = "fp64",
+                  convention: str = "ref", cache: Optional[str] = None,
+                  algorithm: str = "fft") -> Tuple[str, int, int, float]:
+    """Measured best (algorithm, tile, fuse, ms per run)."""
     lib = load_library()
     w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).ravel())
     p = Problem()
@@ -274,9 +288,11 @@ def autotune(dims: int, sizes: Sequence[int], weights, radius: int, steps: int, 
     p.weights = _dptr(w)
     p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     p.convention = CONVENTIONS[convention] if isinstance(convention, str) else int(convention)
+    p.algorithm = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
     ms = ctypes.c_double()
     _check(lib.ffs_autotune(ctypes.byref(p), cache.encode() if cache else None, ctypes.byref(ms)))
-    return p.tile, p.fuse, ms.value
+    name = {v: k for k, v in ALGORITHMS.items()}[p.algorithm]
+    return name, p.tile, p.fuse, ms.value
 
 
 class Batch:

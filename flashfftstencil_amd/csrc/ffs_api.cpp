@@ -65,6 +65,8 @@ struct ffs_plan {
   ffs_problem prob{};
   std::vector<double> weights;
   int kind = 0;
+  bool direct = false;  // FFS_ALGO_DIRECT path (no transform tables)
+  bool star = false;    // direct path: taps off the axes are all zero
   int dev = 0;
   int num_cus = 256;
   int32_t P[3] = {1, 1, 1};      // DFT length per axis (1D: 240 on axis "1")
@@ -118,6 +120,8 @@ ffs_status validate(const ffs_problem* p, int64_t* npoints) {
   if (p->convention != FFS_CONV_REF && p->convention != FFS_CONV_CENTRED)
     return fail(FFS_ERR_INVALID_ARG, "unknown convention %d", p->convention);
   if (p->fuse < 0) return fail(FFS_ERR_INVALID_ARG, "fuse must be >= 0 (got %d)", p->fuse);
+  if (p->algorithm < FFS_ALGO_FFT || p->algorithm > FFS_ALGO_AUTO)
+    return fail(FFS_ERR_INVALID_ARG, "unknown algorithm %d", p->algorithm);
   int64_t n = 1;
   for (int d = 0; d < p->dims; d++) {
     if (p->sizes[d] < 1) return fail(FFS_ERR_INVALID_ARG, "size[%d] must be >= 1 (got %lld)", d, (long long)p->sizes[d]);
@@ -133,10 +137,95 @@ ffs_status validate(const ffs_problem* p, int64_t* npoints) {
   return FFS_OK;
 }
 
+// Split T into blocks of depth t (balanced when t was chosen automatically) plus a tail.
+void set_blocks(ffs_plan* pl, int64_t tbest, bool forced) {
+  const int64_t T = pl->prob.steps;
+  if (T == 0) {
+    pl->t = tbest;
+    pl->nblocks = 0;
+    pl->t_tail = 0;
+    return;
+  }
+  if (tbest > T) tbest = T;
+  int64_t nb = (T + tbest - 1) / tbest;
+  if (!forced) {  // same number of blocks, shallowest depth that still covers T
+    tbest = (T + nb - 1) / nb;
+    nb = (T + tbest - 1) / tbest;
+  }
+  pl->t = tbest;
+  pl->nblocks = nb;
+  const int64_t tail = T - (nb - 1) * tbest;
+  pl->t_tail = (tail == tbest) ? 0 : tail;
+}
+
+// Taps off the axes all zero (Heat-2D, Star2D9P, Heat-3D and custom weights of that form)?
+bool star_support(const ffs_plan* pl) {
+  const ffs_problem& p = pl->prob;
+  const int r = p.radius, K = 2 * r + 1;
+  if (p.dims == 1 || r == 0) return false;
+  for (int64_t i = 0; i < (int64_t)pl->weights.size(); i++) {
+    int off = 0;
+    int64_t q = i;
+    for (int d = 0; d < p.dims; d++, q /= K) off += (q % K) != r;
+    if (off > 1 && pl->weights[i] != 0.0) return false;
+  }
+  return true;
+}
+
+// Direct path: fused depth per launch.  Per point-step cost max(compute, hbm): FMAs x window/valid area
+// x pipeline fill along axis 0, at ~24e12 FP64 FMA/s; 16 B x window/valid area / t at ~5e12 B/s.
+ffs_status choose_config_direct(ffs_plan* pl) {
+  const ffs_problem& p = pl->prob;
+  const int r = p.radius;
+  pl->star = star_support(pl);
+  const int tmax = ffsk::direct_max_depth(p.dims, r, pl->star);
+  if (tmax < 1) return fail(FFS_ERR_UNSUPPORTED, "direct algorithm: radius %d not offered in %dD", r, p.dims);
+  pl->direct = true;
+  pl->kind = -1;
+  int64_t tbest;
+  if (p.fuse > 0) {
+    if (p.fuse > tmax)
+      return fail(FFS_ERR_UNSUPPORTED, "direct algorithm: fuse %d too deep for radius %d in %dD (max %d)", p.fuse, r,
+                  p.dims, tmax);
+    tbest = p.fuse;
+  } else if (p.dims == 1) {
+    tbest = tmax;
+  } else {
+    const int K = 2 * r + 1;
+    double taps = p.dims == 2 ? K * K : K * K * K;
+    if (pl->star) taps = 1 + 2 * r * p.dims;
+    double best = 1e300;
+    tbest = 1;
+    for (int t = 1; t <= tmax; t++) {
+      const double area = p.dims == 2
+                              ? (double)ffsk::kDir2Cols / (ffsk::kDir2Cols - 2 * r * t)
+                              : (double)ffsk::kDir3Rows * ffsk::kDir3Cols /
+                                    ((ffsk::kDir3Rows - 2 * r * t) * (ffsk::kDir3Cols - 2 * r * t));
+      const double fill = 1.0 + (2.0 * r * t + t - 1) / 256.0;
+      const double cost = std::max(taps * area * fill / 24e12, 16.0 * area / t / 5e12);
+      if (cost < best * (1 - 1e-9)) {
+        best = cost;
+        tbest = t;
+      }
+    }
+  }
+  set_blocks(pl, tbest, p.fuse > 0);
+  return FFS_OK;
+}
+
 // Choose tile family, fused depth and valid widths.
 ffs_status choose_config(ffs_plan* pl) {
   const ffs_problem& p = pl->prob;
   const int r = p.radius;
+  if (p.algorithm == FFS_ALGO_DIRECT && p.precision != FFS_PREC_FP64)
+    return fail(FFS_ERR_UNSUPPORTED, "the direct algorithm computes in FP64: use FFS_PREC_FP64");
+  if (p.algorithm == FFS_ALGO_DIRECT) return choose_config_direct(pl);
+  // AUTO: direct in 2D/3D (measured on MI355X: Box2D25P 8192^2 601 vs 152 GStencils/s FP64 FFT, Box2D9P 1323
+  // vs 324, Box3D27P 512^3 289 vs 121), FFT in 1D (Heat-1D 2^20 x 64: 240-point PFA 1784 vs direct 1404).
+  if (p.algorithm == FFS_ALGO_AUTO && p.dims > 1 && ffsk::direct_max_depth(p.dims, r, star_support(pl)) > 0) {
+    pl->prob.precision = FFS_PREC_FP64;
+    return choose_config_direct(pl);
+  }
   if (p.dims == 1) {
     if (p.tile != 0 && p.tile != 240)
       return fail(FFS_ERR_UNSUPPORTED, "1D tile must be 240 (15x16 prime-factor window), got %d", p.tile);
@@ -189,23 +278,7 @@ ffs_status choose_config(ffs_plan* pl) {
       }
     }
   }
-  const int64_t T = p.steps;
-  if (T == 0) {
-    pl->t = tbest;
-    pl->nblocks = 0;
-    pl->t_tail = 0;
-  } else {
-    if (tbest > T) tbest = T;
-    int64_t nb = (T + tbest - 1) / tbest;
-    if (p.fuse == 0) {  // balance: same number of blocks, shallowest depth that still covers T
-      tbest = (T + nb - 1) / nb;
-      nb = (T + tbest - 1) / tbest;
-    }
-    pl->t = tbest;
-    pl->nblocks = nb;
-    const int64_t tail = T - (nb - 1) * tbest;
-    pl->t_tail = (tail == tbest) ? 0 : tail;
-  }
+  set_blocks(pl, tbest, p.fuse > 0);
   const int hi = p.convention == FFS_CONV_REF ? 2 * r : r;  // largest tap offset of one step
   for (int d = 0; d < 3; d++) pl->V[d] = 1;
   if (p.dims == 1) {
@@ -273,6 +346,68 @@ ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, boo
   return a;
 }
 
+// Direct path: launch geometry of one block of depth t (the window is fixed, the valid part shrinks
+// with t).  About two workgroups per CU along axis 0, each streaming a segment of rows (planes).
+ffsk::DirectArgs make_direct_args(const ffs_plan* pl, const double* src, double* dst, int64_t t) {
+  const ffs_problem& p = pl->prob;
+  const int r = p.radius, K = 2 * r + 1;
+  ffsk::DirectArgs a{};
+  a.src = src;
+  a.dst = dst;
+  a.n0 = p.sizes[0];
+  a.n1 = p.dims > 1 ? p.sizes[1] : 1;
+  a.n2 = p.dims > 2 ? p.sizes[2] : 1;
+  a.steps = (int32_t)t;
+  a.lo = p.convention == FFS_CONV_REF ? 2 * r : r;
+  // window-order taps: g[e] multiplies u[i + e]; ref convention flips every axis (w[2r - e])
+  const int64_t ntaps = (int64_t)pl->weights.size();
+  for (int64_t i = 0; i < ntaps; i++) {
+    int64_t src_i = i;
+    if (p.convention == FFS_CONV_REF) {
+      src_i = 0;
+      int64_t q = i, mul = 1;
+      for (int d = 0; d < p.dims; d++, q /= K, mul *= K) src_i += (2 * r - q % K) * mul;
+    }
+    a.g[i] = pl->weights[src_i];
+  }
+  auto cdiv = [](int64_t x, int64_t y) { return (x + y - 1) / y; };
+  const int64_t shrink = 2 * (int64_t)r * t;
+  if (p.dims == 1) {
+    a.valid1 = (int32_t)(ffsk::kDir1Window - shrink);
+    a.nseg = (int32_t)cdiv(a.n0, a.valid1);
+    a.strips1 = a.strips2 = 1;
+    a.seg = a.n0;
+    return a;
+  }
+  int64_t strips;
+  if (p.dims == 2) {
+    a.valid1 = (int32_t)(ffsk::kDir2Cols - shrink);
+    a.strips1 = (int32_t)cdiv(a.n1, a.valid1);
+    a.strips2 = 1;
+    strips = a.strips1;
+  } else {
+    a.valid1 = (int32_t)(ffsk::kDir3Rows - shrink);
+    a.valid2 = (int32_t)(ffsk::kDir3Cols - shrink);
+    a.strips1 = (int32_t)cdiv(a.n1, a.valid1);
+    a.strips2 = (int32_t)cdiv(a.n2, a.valid2);
+    strips = (int64_t)a.strips1 * a.strips2;
+  }
+  int64_t nseg = std::max<int64_t>(1, (2 * (int64_t)pl->num_cus + strips / 2) / strips);
+  nseg = std::min<int64_t>(nseg, cdiv(a.n0, 16));
+  a.seg = cdiv(a.n0, nseg);
+  a.nseg = (int32_t)cdiv(a.n0, a.seg);
+  return a;
+}
+
+hipError_t launch_one_block(ffs_plan* pl, const double* src, double* dst, bool tail, hipStream_t s) {
+  if (pl->direct) {
+    const ffsk::DirectArgs a = make_direct_args(pl, src, dst, tail ? pl->t_tail : pl->t);
+    return ffsk::launch_direct(a, pl->prob.dims, pl->prob.radius, pl->star, s);
+  }
+  ffsk::TileArgs a = make_args(pl, src, dst, tail);
+  return ffsk::launch_block(a, pl->kind, s, pl->num_cus);
+}
+
 // Enqueue the whole block chain d_in -> ... -> d_out on stream s.
 ffs_status enqueue_chain(ffs_plan* pl, const double* d_in, double* d_out, hipStream_t s) {
   const int64_t nb = pl->nblocks;
@@ -280,9 +415,8 @@ ffs_status enqueue_chain(ffs_plan* pl, const double* d_in, double* d_out, hipStr
   for (int64_t b = 0; b < nb; b++) {
     double* dst = ((nb - 1 - b) % 2 == 0) ? d_out : pl->d_scratch;
     const bool tail = (b == nb - 1) && pl->t_tail > 0;
-    ffsk::TileArgs a = make_args(pl, src, dst, tail);
-    hipError_t e = ffsk::launch_block(a, pl->kind, s, pl->num_cus);
-    if (e != hipSuccess) return fail(FFS_ERR_HIP, "tile kernel launch failed: %s", hipGetErrorString(e));
+    hipError_t e = launch_one_block(pl, src, dst, tail, s);
+    if (e != hipSuccess) return fail(FFS_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
     src = dst;
   }
   return FFS_OK;
@@ -414,11 +548,22 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
     return fail(FFS_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950 only", pl->dev, prop.gcnArchName);
   }
   pl->num_cus = prop.multiProcessorCount;
-  pl->hlen = ffsk::spectrum_len(pl->kind);
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) {
     if (x != hipSuccess && e == hipSuccess) e = x;
   };
+  if (pl->direct) {  // no transform tables: the taps travel in the kernel arguments
+    if (pl->nblocks >= 2) chk(hipMalloc(&pl->d_scratch, sizeof(double) * npoints));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&pl->cap_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      release(pl);
+      return fail(e == hipErrorOutOfMemory ? FFS_ERR_OUT_OF_MEMORY : FFS_ERR_HIP, "plan set-up failed: %s",
+                  hipGetErrorString(e));
+    }
+    *out = pl;
+    return FFS_OK;
+  }
+  pl->hlen = ffsk::spectrum_len(pl->kind);
   chk(hipMalloc(&pl->d_ops, sizeof(double) * ffsk::kOpsLen));
   if (p->precision != FFS_PREC_FP64)
     chk(hipMalloc(&pl->d_opsm, sizeof(uint32_t) * std::max(ffsk::kOpsMixedLen, ffsk::kOps32Words)));
@@ -472,6 +617,45 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   info->steps = p.steps;
   info->precision = p.precision;
   info->convention = p.convention;
+  info->algorithm = pl->direct ? FFS_ALGO_DIRECT : FFS_ALGO_FFT;
+  if (pl->direct) {
+    const ffsk::DirectArgs a = make_direct_args(pl, nullptr, nullptr, pl->t);
+    const int r = p.radius, K = 2 * r + 1;
+    const int64_t halo = 2 * (int64_t)r * pl->t;
+    double taps = std::pow((double)K, p.dims);
+    if (pl->star) taps = 1 + 2 * r * p.dims;
+    for (int d = 0; d < 3; d++) info->tile[d] = info->valid[d] = 1;
+    double computed;  // points the launch computes, all levels (loop rounding to 2r+1 rows ignored)
+    if (p.dims == 1) {
+      info->tile[0] = ffsk::kDir1Window;
+      info->valid[0] = a.valid1;
+      info->tiles = a.nseg;
+      computed = (double)a.nseg * ffsk::kDir1Window * pl->t;
+    } else {
+      const double rows = (double)a.n0 + (double)a.nseg * (halo + pl->t - 1);
+      info->tile[0] = (int32_t)(a.seg + halo);
+      info->valid[0] = (int32_t)a.seg;
+      if (p.dims == 2) {
+        info->tile[1] = ffsk::kDir2Cols;
+        info->valid[1] = a.valid1;
+        info->tiles = (int64_t)a.nseg * a.strips1;
+        computed = rows * a.strips1 * ffsk::kDir2Cols * pl->t;
+      } else {
+        info->tile[1] = ffsk::kDir3Rows;
+        info->tile[2] = ffsk::kDir3Cols;
+        info->valid[1] = a.valid1;
+        info->valid[2] = a.valid2;
+        info->tiles = (int64_t)a.nseg * a.strips1 * a.strips2;
+        computed = rows * a.strips1 * a.strips2 * ffsk::kDir3Rows * ffsk::kDir3Cols * pl->t;
+      }
+    }
+    info->fuse = (int32_t)pl->t;
+    info->fuse_tail = (int32_t)pl->t_tail;
+    info->blocks = info->launches = pl->nblocks;
+    info->flops_per_block = 2.0 * taps * computed;
+    info->bytes_per_block = 16.0 * (double)pl->npoints;
+    return FFS_OK;
+  }
   if (p.dims == 1) {
     info->tile[0] = 240;
     info->valid[0] = pl->V[1];
@@ -560,9 +744,8 @@ ffs_status ffs_run_block(ffs_plan* pl, const double* d_in, double* d_out, int32_
   if (pl->nblocks == 0) return fail(FFS_ERR_INVALID_ARG, "plan has no blocks (T = 0)");
   FFS_HIP(hipSetDevice(pl->dev));
   const bool use_tail = tail != 0 && pl->t_tail > 0;
-  ffsk::TileArgs a = make_args(pl, d_in, d_out, use_tail);
-  hipError_t e = ffsk::launch_block(a, pl->kind, (hipStream_t)stream, pl->num_cus);
-  if (e != hipSuccess) return fail(FFS_ERR_HIP, "tile kernel launch failed: %s", hipGetErrorString(e));
+  hipError_t e = launch_one_block(pl, d_in, d_out, use_tail, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(FFS_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
   return FFS_OK;
 }
 
@@ -607,41 +790,72 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
   ffs_status st = validate(p, &npoints);
   if (st != FFS_OK) return st;
   char key[256];
-  std::snprintf(key, sizeof key, "d%d_r%d_p%d_c%d_n%lldx%lldx%lld_T%lld", p->dims, p->radius, p->precision,
-                p->convention, (long long)p->sizes[0], (long long)(p->dims > 1 ? p->sizes[1] : 1),
+  std::snprintf(key, sizeof key, "d%d_r%d_p%d_c%d_a%d_n%lldx%lldx%lld_T%lld", p->dims, p->radius, p->precision,
+                p->convention, p->algorithm, (long long)p->sizes[0], (long long)(p->dims > 1 ? p->sizes[1] : 1),
                 (long long)(p->dims > 2 ? p->sizes[2] : 1), (long long)p->steps);
-  if (cache_path) {  // a line: {"key": "...", "tile": P, "fuse": t, "ms": x}
+  auto apply = [&](int algo, int tile, int fuse) {
+    p->algorithm = algo;
+    if (algo == FFS_ALGO_DIRECT) p->precision = FFS_PREC_FP64;
+    p->tile = algo == FFS_ALGO_DIRECT ? 0 : tile;
+    p->fuse = fuse;
+  };
+  if (cache_path) {  // a line: {"key": "...", "algorithm": A, "tile": P, "fuse": t, "ms": x}
     if (FILE* f = std::fopen(cache_path, "r")) {
       char line[512];
-      int tile = -1, fuse = -1;
+      int algo = -1, tile = -1, fuse = -1;
       double ms = 0;
       while (std::fgets(line, sizeof line, f)) {
         char k[256];
-        int tl, fu;
+        int al, tl, fu;
         double m;
-        if (std::sscanf(line, " {\"key\": \"%255[^\"]\", \"tile\": %d, \"fuse\": %d, \"ms\": %lf", k, &tl, &fu, &m) == 4 &&
+        if (std::sscanf(line, " {\"key\": \"%255[^\"]\", \"algorithm\": %d, \"tile\": %d, \"fuse\": %d, \"ms\": %lf", k,
+                        &al, &tl, &fu, &m) == 5 &&
             std::strcmp(k, key) == 0) {
+          algo = al;
           tile = tl;
           fuse = fu;
           ms = m;
         }
       }
       std::fclose(f);
-      if (tile >= 0) {
-        p->tile = tile;
-        p->fuse = fuse;
+      if (algo >= 0) {
+        apply(algo, tile, fuse);
         if (best_ms) *best_ms = ms;
         return FFS_OK;
       }
     }
   }
-  std::vector<int> tiles;
-  if (p->dims == 1) tiles = {240};
-  else if (p->dims == 3) tiles = {16};
-  else if (p->precision == FFS_PREC_MIXED_BF16 || p->precision == FFS_PREC_MIXED_BF16X2) tiles = {16, 32};
-  else tiles = {16};
+  struct Cand {
+    int algo, tile, fuse;
+  };
+  std::vector<Cand> cands;
+  const int64_t tcap = std::max<int64_t>(p->steps, 1);
+  auto pick = [&](int64_t t, int64_t tmax) { return t <= 8 || (t & (t - 1)) == 0 || t == tmax; };
+  if (p->algorithm != FFS_ALGO_DIRECT) {  // FFT candidates: every tile of the form, depths up to 8, then 2^k + max
+    std::vector<int> tiles;
+    if (p->dims == 1) tiles = {240};
+    else if (p->dims == 3) tiles = {16};
+    else if (p->precision == FFS_PREC_MIXED_BF16 || p->precision == FFS_PREC_MIXED_BF16X2) tiles = {16, 32};
+    else tiles = {16};
+    for (int tile : tiles) {
+      const int vmin = p->dims == 1 ? 16 : 2;
+      const int64_t tmax = p->radius == 0 ? 1 : std::max<int64_t>(1, (tile - vmin) / (2 * p->radius));
+      for (int64_t t = 1; t <= std::min(tmax, tcap); t++)
+        if (pick(t, tmax)) cands.push_back({FFS_ALGO_FFT, tile, (int)t});
+    }
+  }
+  if (p->algorithm != FFS_ALGO_FFT) {  // direct candidates (FP64), when the radius is offered
+    ffs_plan probe;
+    probe.prob = *p;
+    int64_t ntaps = 1;
+    for (int d = 0; d < p->dims; d++) ntaps *= 2 * p->radius + 1;
+    probe.weights.assign(p->weights, p->weights + ntaps);
+    const int64_t tmax = ffsk::direct_max_depth(p->dims, p->radius, star_support(&probe));
+    for (int64_t t = 1; t <= std::min(tmax, tcap); t++)
+      if (pick(t, tmax)) cands.push_back({FFS_ALGO_DIRECT, 0, (int)t});
+  }
   double best = 1e300;
-  int best_tile = 0, best_fuse = 0;
+  Cand win{-1, 0, 0};
   double *din = nullptr, *dout = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   auto cleanup = [&]() {
@@ -650,59 +864,55 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
   };
-  for (int tile : tiles) {
-    const int vmin = p->dims == 1 ? 16 : 2;
-    const int64_t tmax = p->radius == 0 ? 1 : std::max<int64_t>(1, (tile - vmin) / (2 * p->radius));
-    for (int64_t t = 1; t <= std::min<int64_t>(tmax, std::max<int64_t>(p->steps, 1)); t++) {
-      if (t > 8 && (t & (t - 1)) != 0 && t != tmax) continue;  // all depths up to 8, then powers of two + max
-      ffs_problem c = *p;
-      c.tile = tile;
-      c.fuse = (int32_t)t;
-      ffs_plan* pl = nullptr;
-      if (ffs_plan_create(&c, &pl) != FFS_OK) continue;
-      if (!din) {
-        hipError_t e = hipMalloc(&din, sizeof(double) * npoints);
-        if (e == hipSuccess) e = hipMalloc(&dout, sizeof(double) * npoints);
-        if (e == hipSuccess) e = hipMemset(din, 0, sizeof(double) * npoints);
-        if (e == hipSuccess) e = hipEventCreate(&e0);
-        if (e == hipSuccess) e = hipEventCreate(&e1);
-        if (e != hipSuccess) {
-          ffs_plan_destroy(pl);
-          cleanup();
-          return fail(FFS_ERR_OUT_OF_MEMORY, "autotune scratch: %s", hipGetErrorString(e));
-        }
-      }
-      double ms_best = 1e300;
-      for (int rep = 0; rep < 3; rep++) {  // rep 0 captures the graph and warms up
-        (void)hipEventRecord(e0, nullptr);
-        st = ffs_run(pl, din, dout, nullptr);
-        (void)hipEventRecord(e1, nullptr);
-        (void)hipEventSynchronize(e1);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (st != FFS_OK) break;
-        if (rep > 0) ms_best = std::min(ms_best, (double)ms);
-      }
-      ffs_plan_destroy(pl);
-      if (st != FFS_OK) {
+  for (const Cand& cd : cands) {
+    ffs_problem c = *p;
+    c.algorithm = cd.algo;
+    if (cd.algo == FFS_ALGO_DIRECT) c.precision = FFS_PREC_FP64;
+    c.tile = cd.tile;
+    c.fuse = cd.fuse;
+    ffs_plan* pl = nullptr;
+    if (ffs_plan_create(&c, &pl) != FFS_OK) continue;
+    if (!din) {
+      hipError_t e = hipMalloc(&din, sizeof(double) * npoints);
+      if (e == hipSuccess) e = hipMalloc(&dout, sizeof(double) * npoints);
+      if (e == hipSuccess) e = hipMemset(din, 0, sizeof(double) * npoints);
+      if (e == hipSuccess) e = hipEventCreate(&e0);
+      if (e == hipSuccess) e = hipEventCreate(&e1);
+      if (e != hipSuccess) {
+        ffs_plan_destroy(pl);
         cleanup();
-        return st;
+        return fail(FFS_ERR_OUT_OF_MEMORY, "autotune scratch: %s", hipGetErrorString(e));
       }
-      if (ms_best < best) {
-        best = ms_best;
-        best_tile = tile;
-        best_fuse = (int)t;
-      }
+    }
+    double ms_best = 1e300;
+    for (int rep = 0; rep < 3; rep++) {  // rep 0 captures the graph and warms up
+      (void)hipEventRecord(e0, nullptr);
+      st = ffs_run(pl, din, dout, nullptr);
+      (void)hipEventRecord(e1, nullptr);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (st != FFS_OK) break;
+      if (rep > 0) ms_best = std::min(ms_best, (double)ms);
+    }
+    ffs_plan_destroy(pl);
+    if (st != FFS_OK) {
+      cleanup();
+      return st;
+    }
+    if (ms_best < best) {
+      best = ms_best;
+      win = cd;
     }
   }
   cleanup();
-  if (best_fuse == 0) return fail(FFS_ERR_UNSUPPORTED, "no admissible configuration");
-  p->tile = best_tile;
-  p->fuse = best_fuse;
+  if (win.algo < 0) return fail(FFS_ERR_UNSUPPORTED, "no admissible configuration");
+  apply(win.algo, win.tile, win.fuse);
   if (best_ms) *best_ms = best;
   if (cache_path) {
     if (FILE* f = std::fopen(cache_path, "a")) {
-      std::fprintf(f, "{\"key\": \"%s\", \"tile\": %d, \"fuse\": %d, \"ms\": %.6f}\n", key, best_tile, best_fuse, best);
+      std::fprintf(f, "{\"key\": \"%s\", \"algorithm\": %d, \"tile\": %d, \"fuse\": %d, \"ms\": %.6f}\n", key,
+                   win.algo, win.tile, win.fuse, best);
       std::fclose(f);
     }
   }
@@ -716,6 +926,8 @@ ffs_status ffs_batch_create(const ffs_problem* problems, int32_t count, ffs_batc
   const int32_t prec = problems[0].precision;
   for (int32_t i = 0; i < count; i++) {
     if (problems[i].dims != 2) return fail(FFS_ERR_UNSUPPORTED, "batch problem %d: only 2D grids are batched", i);
+    if (problems[i].algorithm != FFS_ALGO_FFT)
+      return fail(FFS_ERR_UNSUPPORTED, "batch problem %d: batches run the FFT algorithm only", i);
     if (problems[i].precision != prec)
       return fail(FFS_ERR_INVALID_ARG, "batch problem %d: all problems must share one precision", i);
     if (prec != FFS_PREC_MIXED_BF16 && prec != FFS_PREC_MIXED_BF16X2)

@@ -56,6 +56,29 @@ struct TileArgs {
   double dc_gain;  // K(0)^t: gain of the constant mode (mixed forms add mean * gain back in FP64)
 };
 
+// Direct (non-FFT) FP64 kernels with temporal blocking (ffs_direct.hip).
+constexpr int kDir1Pts = 8;                    // 1D: points per thread, in registers
+constexpr int kDir1Window = 256 * kDir1Pts;    // 1D: window per workgroup
+constexpr int kDir1MaxDepth = 64;              // 1D: largest fused depth of one launch
+constexpr int kDir2Cols = 512;                 // 2D: window columns per workgroup (2 per thread)
+constexpr int kDir3Rows = 64, kDir3Cols = 32;  // 3D: plane window (axis 1 x axis 2), 4 x 2 per thread
+
+struct DirectArgs {
+  const double* src;
+  double* dst;
+  int64_t n0, n1, n2;        // grid sides (1D: n0 = n)
+  int64_t seg;               // 2D/3D: output rows (planes) along axis 0 per workgroup
+  int32_t nseg;              // 2D/3D: workgroups along axis 0; 1D: workgroups
+  int32_t strips1, strips2;  // workgroups along axes 1 and 2
+  int32_t valid1, valid2;    // output width per workgroup along axes 1 and 2 (1D: valid1)
+  int32_t lo;                // per-step window offset: 2r (ref convention) or r (centred)
+  int32_t steps;             // fused depth of this launch
+  double g[49];              // taps in window order, g[e] multiplies u[i + e], e in [0, 2r]^d row-major
+};
+
+int direct_max_depth(int dims, int radius, bool star);  // 0 = radius not offered
+hipError_t launch_direct(const DirectArgs& a, int dims, int radius, bool star, hipStream_t s);
+
 // Launch helpers implemented in ffs_kernels.hip.
 hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hipStream_t s);
 hipError_t gen_operands32(uint32_t* ops32, hipStream_t s);

@@ -50,6 +50,8 @@ static void print_help() {
       "                       r is the shape's radius\n"
       "  --precision P        fp64 (default) | bf16 (mixed: BF16 split operands) | f32 (mixed: FP32 operands)\n"
       "  --convention C       ref (default, out[p] = sum_a w[a] in[p-a]) | centred (out[p] = sum_o w[o+r] in[p+o])\n"
+      "  --algorithm A        fft (default: tiled FFT stencil) | direct (FP64 stencil sum, temporal blocking) |\n"
+      "                       auto (direct where offered)\n"
       "  --seed S             seed of the weights and the initial grid (default 0x5EED0000)\n"
       "  --fuse t             fused time depth (default automatic)\n"
       "  --no-graph           plain launches instead of a hipGraph replay\n"
@@ -76,7 +78,7 @@ static bool parse_i64(const char* s, long long* v) {
 
 int main(int argc, char** argv) {
   std::vector<std::string> pos;
-  std::string custom, prec = "fp64", conv = "ref";
+  std::string custom, prec = "fp64", conv = "ref", algo = "fft";
   unsigned long long seed = 0x5EED0000ULL;
   int fuse = 0, use_graph = 1;
   bool tune = false;
@@ -100,6 +102,8 @@ int main(int argc, char** argv) {
       prec = need("--precision");
     } else if (a == "--convention") {
       conv = need("--convention");
+    } else if (a == "--algorithm") {
+      algo = need("--algorithm");
     } else if (a == "--seed") {
       seed = std::strtoull(need("--seed"), nullptr, 0);
     } else if (a == "--fuse") {
@@ -183,6 +187,7 @@ int main(int argc, char** argv) {
   p.steps = T;
   p.precision = prec == "fp64" ? FFS_PREC_FP64 : prec == "bf16" ? FFS_PREC_MIXED_BF16 : prec == "f32" ? FFS_PREC_MIXED_F32 : -1;
   p.convention = conv == "ref" ? FFS_CONV_REF : conv == "centred" || conv == "centered" ? FFS_CONV_CENTRED : -1;
+  p.algorithm = algo == "fft" ? FFS_ALGO_FFT : algo == "direct" ? FFS_ALGO_DIRECT : algo == "auto" ? FFS_ALGO_AUTO : -1;
   p.fuse = fuse;
   p.use_graph = use_graph;
 

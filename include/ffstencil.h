@@ -65,6 +65,19 @@ typedef enum ffs_precision {
  *   CENTRED: out[p] = sum_{o in [-r,r]^d} w[o + r] * in[(p + o) mod n]   (src/1D/check_correct.hpp:15-38) */
 typedef enum ffs_convention { FFS_CONV_REF = 0, FFS_CONV_CENTRED = 1 } ffs_convention;
 
+/* How the T steps are computed.
+ *   FFT:    the tiled FFT-stencil path (DFT-as-GEMM on the matrix cores, spectrum power K^t), in the
+ *           requested precision.  The drop-in for the reference's kernels (src/2D/rfft_2d/2d_rfft_1_async.cu).
+ *   DIRECT: the stencil sum itself in FP64 FMAs, t steps per launch through LDS/registers (temporal
+ *           blocking); precision must be FFS_PREC_FP64.  No kernel counterpart in the reference (its
+ *           CPU check, src/2D/check_correct.hpp:64-100, is the semantics): a fallback and a baseline.
+ *           Radius <= 3 in 1D and 2D, <= 1 in 3D.
+ *   AUTO:   DIRECT in 2D and 3D where it is offered, else FFT.  On MI355X the FP64 vector and FP64 matrix
+ *           peaks are equal, so the direct sum, with several times fewer FP64 operations per point-step
+ *           than the transform path at these radii, is the faster one there; in 1D the 240-point
+ *           prime-factor FFT tile is faster.  A DIRECT result is FP64 whatever precision was asked. */
+typedef enum ffs_algorithm { FFS_ALGO_FFT = 0, FFS_ALGO_DIRECT = 1, FFS_ALGO_AUTO = 2 } ffs_algorithm;
+
 typedef struct ffs_problem {
   int32_t dims;          /* 1, 2 or 3 */
   int32_t radius;        /* r >= 0; taps per axis K = 2r+1 */
@@ -77,7 +90,8 @@ typedef struct ffs_problem {
   int32_t fuse;          /* fused depth t; 0 = automatic */
   int32_t use_graph;     /* 1 = replay the block loop from a hipGraph, 0 = plain launches */
   int32_t device;        /* HIP device ordinal */
-  int32_t reserved[4];
+  int32_t algorithm;     /* ffs_algorithm (default FFS_ALGO_FFT) */
+  int32_t reserved[3];
 } ffs_problem;
 
 /* What a plan chose. */
@@ -86,15 +100,18 @@ typedef struct ffs_plan_info {
   int64_t sizes[3];
   int64_t steps;
   int32_t precision, convention;
-  int32_t tile[3];        /* DFT length per axis (1 for unused axes) */
-  int32_t valid[3];       /* valid output width per tile and axis, V = P - 2 r t */
+  int32_t tile[3];        /* FFT: DFT length per axis; DIRECT: input window per workgroup (1 = unused axis) */
+  int32_t valid[3];       /* output width per tile (workgroup) and axis: V = P - 2 r t */
   int32_t fuse;           /* t */
   int32_t fuse_tail;      /* T mod t (0 = no tail block) */
   int64_t blocks;         /* fused blocks per run, tail included */
   int64_t tiles;          /* tiles per block */
   int64_t launches;       /* kernel launches per run */
-  double flops_per_block; /* matrix-core flops one block executes (see header comment of the kernels) */
+  double flops_per_block; /* flops one block executes: matrix-core flops (FFT; see the kernels' header
+                             comment) or FP64 FMA flops including the halo recomputation (DIRECT) */
   double bytes_per_block; /* algorithmic HBM bytes of one block: 16 B per grid point */
+  int32_t algorithm;      /* ffs_algorithm actually used (FFT or DIRECT) */
+  int32_t reserved[3];
 } ffs_plan_info;
 
 typedef struct ffs_plan ffs_plan;
@@ -130,11 +147,13 @@ ffs_status ffs_run_host(ffs_plan* plan, const double* h_in, double* h_out);
  * d_out = S^t(d_in).  One kernel launch; used to time the tile kernel alone. */
 ffs_status ffs_run_block(ffs_plan* plan, const double* d_in, double* d_out, int32_t tail, void* stream);
 
-/* Autotuner: time the admissible (tile, fused depth) plans of `p` on its device (full T-step runs on a
- * scratch grid of p's sizes) and write the fastest into p->tile and p->fuse.  With a non-NULL
- * `cache_path`, results are read from / appended to that JSON-lines file keyed by (dims, radius,
- * precision, convention, sizes, T), so each configuration is measured once.  best_ms (may be NULL)
- * receives the winner's time per run.  No reference counterpart (the reference hard-codes its tile). */
+/* Autotuner: time the admissible (algorithm, tile, fused depth) plans of `p` on its device (full
+ * T-step runs on a scratch grid of p's sizes) and write the fastest into p->tile, p->fuse and
+ * p->algorithm (FFS_ALGO_AUTO searches both algorithms; a DIRECT winner also sets p->precision to
+ * FFS_PREC_FP64).  With a non-NULL `cache_path`, results are read from / appended to that JSON-lines
+ * file keyed by (dims, radius, precision, convention, algorithm, sizes, T), so each configuration is
+ * measured once.  best_ms (may be NULL) receives the winner's time per run.  No reference counterpart
+ * (the reference hard-codes its tile). */
 ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms);
 
 /* Batches of independent 2D grids (different sizes, stencils and T).  Block b of every grid that still
