@@ -172,8 +172,10 @@ bool star_support(const ffs_plan* pl) {
   return true;
 }
 
-// Direct path: fused depth per launch.  Per point-step cost max(compute, hbm): FMAs x window/valid area
-// x pipeline fill along axis 0, at ~24e12 FP64 FMA/s; 16 B x window/valid area / t at ~5e12 B/s.
+// Direct path: fused depth per launch.  Per point-step cost max(compute, memory): FMAs x window/valid area
+// x pipeline fill along axis 0 at ~24e12 FP64 FMA/s, against 16 B x window/valid area / t at an effective
+// 2.5e12 B/s (fitted to MI355X timings of t = 1..6: the row stream is latency- rather than
+// bandwidth-limited, so deeper fusion pays longer than the 8 TB/s peak suggests).
 ffs_status choose_config_direct(ffs_plan* pl) {
   const ffs_problem& p = pl->prob;
   const int r = p.radius;
@@ -202,7 +204,7 @@ ffs_status choose_config_direct(ffs_plan* pl) {
                               : (double)ffsk::kDir3Rows * ffsk::kDir3Cols /
                                     ((ffsk::kDir3Rows - 2 * r * t) * (ffsk::kDir3Cols - 2 * r * t));
       const double fill = 1.0 + (2.0 * r * t + t - 1) / 256.0;
-      const double cost = std::max(taps * area * fill / 24e12, 16.0 * area / t / 5e12);
+      const double cost = std::max(taps * area * fill / 24e12, 16.0 * area / t / 2.5e12);
       if (cost < best * (1 - 1e-9)) {
         best = cost;
         tbest = t;

@@ -108,8 +108,11 @@ __global__ __launch_bounds__(256) void k_direct2d(DirectArgs a) {
   for (int s = 0; s < T; s++)
 #pragma unroll
     for (int k = 0; k < U; k++) acc[s][k][0] = acc[s][k][1] = 0.0;
-  int64_t yr = dwrap(y0 - (int64_t)T * a.lo, n0);  // grid row of the input row of the current iteration
+  // input rows are loaded two iterations ahead of their LDS write (one row ahead leaves HBM latency exposed)
+  int64_t yr = dwrap(y0 - (int64_t)T * a.lo, n0);
   double p0 = a.src[yr * n1 + col0], p1 = a.src[yr * n1 + col1];
+  yr = yr + 1 == n0 ? 0 : yr + 1;
+  double q0 = a.src[yr * n1 + col0], q1 = a.src[yr * n1 + col1];
   constexpr int64_t kShift = 2 * R * T + T - 1;  // level t emits row m - kShift in iteration m
   const int64_t M = rows + kShift;
   const bool st0 = c < a.valid1 && x0 + c < n1;
@@ -120,9 +123,11 @@ __global__ __launch_bounds__(256) void k_direct2d(DirectArgs a) {
       const int64_t m = m0 + p;
       const int sl = (int)(m & 1);
       *reinterpret_cast<double2*>(&buf[0][sl][c]) = make_double2(p0, p1);
+      p0 = q0;
+      p1 = q1;
       yr = yr + 1 == n0 ? 0 : yr + 1;
-      p0 = a.src[yr * n1 + col0];
-      p1 = a.src[yr * n1 + col1];
+      q0 = a.src[yr * n1 + col0];
+      q1 = a.src[yr * n1 + col1];
       __syncthreads();
 #pragma unroll
       for (int s = 0; s < T; s++) {  // level s + 1
