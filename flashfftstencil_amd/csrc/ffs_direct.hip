@@ -17,6 +17,8 @@
 // m what level s-1 produced in iteration m-1, so one barrier per iteration orders everything (two LDS
 // slots per level).  The unroll by 2r+1 makes every partial-sum index static.
 //
+// 3D windows are 32 x 64 points per plane, 512 threads.
+//
 // 1D keeps a 2048-point window in registers (8 points per thread); each step exchanges the first 2r
 // values of every thread with its left neighbour through LDS (one barrier per step).
 #include <hip/hip_runtime.h>
@@ -170,16 +172,20 @@ __global__ __launch_bounds__(256) void k_direct2d(DirectArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------------
-// 3D: plane window kDir3Rows x kDir3Cols (axes 1 x 2; 4 x 2 points per thread) streamed along axis 0.
+// 3D: plane window kDir3Rows x kDir3Cols (axes 1 x 2; kDir3PerThread x 2 points per thread) streamed
+// along axis 0.  The window is widest along the contiguous axis: 64 columns (512 B rows) load at
+// ~1.25x the rate of 32-column rows.
 template <int R, int T, bool STAR>
-__global__ __launch_bounds__(256) void k_direct3d(DirectArgs a) {
+__global__ __launch_bounds__(kDir3Threads) void k_direct3d(DirectArgs a) {
   constexpr int U = 2 * R + 1;
-  constexpr int RS = kDir3Cols + 4;   // LDS row stride (doubles)
-  constexpr int PR = kDir3Rows + 4;   // LDS rows per plane (reads run 2r rows past the window)
+  constexpr int MR = kDir3PerThread;
+  constexpr int TX = kDir3Cols / 2;
+  constexpr int RS = kDir3Cols + 4;  // LDS row stride (doubles)
+  constexpr int PR = kDir3Rows + 4;  // LDS rows per plane (reads run 2r rows past the window)
   __shared__ __attribute__((aligned(16))) double buf[T][2][PR * RS];
   const int tid = threadIdx.x;
-  const int tx = tid % 16, ty = tid / 16;
-  const int c = 2 * tx, r0 = 4 * ty;
+  const int tx = tid % TX, ty = tid / TX;
+  const int c = 2 * tx, r0 = MR * ty;
   const int64_t n0 = a.n0, n1 = a.n1, n2 = a.n2;
   const int sx = blockIdx.x % a.strips2;
   const int sy = (blockIdx.x / a.strips2) % a.strips1;
@@ -188,60 +194,66 @@ __global__ __launch_bounds__(256) void k_direct3d(DirectArgs a) {
   const int64_t y0 = (int64_t)sy * a.valid1;  // axis 1
   const int64_t z0 = (int64_t)sz * a.seg;     // axis 0
   const int64_t planes = min(a.seg, n0 - z0);
-  int64_t gcol[2], grow[4];
+  int64_t gcol[2], grow[MR];
 #pragma unroll
   for (int b = 0; b < 2; b++) gcol[b] = dwrap(x0 - (int64_t)T * a.lo + c + b, n2);
 #pragma unroll
-  for (int i = 0; i < 4; i++) grow[i] = dwrap(y0 - (int64_t)T * a.lo + r0 + i, n1) * n2;
-  for (int i = tid; i < T * 2 * PR * RS; i += 256) (&buf[0][0][0])[i] = 0.0;
+  for (int i = 0; i < MR; i++) grow[i] = dwrap(y0 - (int64_t)T * a.lo + r0 + i, n1) * n2;
+  for (int i = tid; i < T * 2 * PR * RS; i += kDir3Threads) (&buf[0][0][0])[i] = 0.0;
   __syncthreads();
-  double acc[T][U][4][2];
+  double acc[T][U][MR][2];
 #pragma unroll
   for (int s = 0; s < T; s++)
 #pragma unroll
     for (int k = 0; k < U; k++)
 #pragma unroll
-      for (int i = 0; i < 4; i++) acc[s][k][i][0] = acc[s][k][i][1] = 0.0;
+      for (int i = 0; i < MR; i++) acc[s][k][i][0] = acc[s][k][i][1] = 0.0;
+  // input planes are loaded two iterations ahead of their LDS write
   int64_t zr = dwrap(z0 - (int64_t)T * a.lo, n0);
-  double pf[4][2];
-  {
+  double pf[MR][2], pg[MR][2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
     const double* pl = a.src + zr * n1 * n2;
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < MR; i++)
 #pragma unroll
-      for (int b = 0; b < 2; b++) pf[i][b] = pl[grow[i] + gcol[b]];
+      for (int b = 0; b < 2; b++) (h == 0 ? pf : pg)[i][b] = pl[grow[i] + gcol[b]];
+    if (h == 0) zr = zr + 1 == n0 ? 0 : zr + 1;
   }
   constexpr int64_t kShift = 2 * R * T + T - 1;
   const int64_t M = planes + kShift;
   bool stc[2];
 #pragma unroll
   for (int b = 0; b < 2; b++) stc[b] = c + b < a.valid2 && x0 + c + b < n2;
-  bool str[4];
+  bool str[MR];
 #pragma unroll
-  for (int i = 0; i < 4; i++) str[i] = r0 + i < a.valid1 && y0 + r0 + i < n1;
+  for (int i = 0; i < MR; i++) str[i] = r0 + i < a.valid1 && y0 + r0 + i < n1;
   for (int64_t m0 = 0; m0 < M; m0 += U) {
 #pragma unroll
     for (int p = 0; p < U; p++) {
       const int64_t m = m0 + p;
       const int sl = (int)(m & 1);
 #pragma unroll
-      for (int i = 0; i < 4; i++)
+      for (int i = 0; i < MR; i++) {
         *reinterpret_cast<double2*>(&buf[0][sl][(r0 + i) * RS + c]) = make_double2(pf[i][0], pf[i][1]);
+        pf[i][0] = pg[i][0];
+        pf[i][1] = pg[i][1];
+      }
       zr = zr + 1 == n0 ? 0 : zr + 1;
       {
         const double* pl = a.src + zr * n1 * n2;
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < MR; i++)
 #pragma unroll
-          for (int b = 0; b < 2; b++) pf[i][b] = pl[grow[i] + gcol[b]];
+          for (int b = 0; b < 2; b++) pg[i][b] = pl[grow[i] + gcol[b]];
       }
       __syncthreads();
 #pragma unroll
       for (int s = 0; s < T; s++) {
         const double* in = s == 0 ? &buf[0][sl][0] : &buf[s][sl ^ 1][0];
-        double v[4 + 2 * R][2 + 2 * R];
+        double v[MR + 2 * R][2 + 2 * R];
 #pragma unroll
-        for (int i = 0; i < 4 + 2 * R; i++)
+        for (int i = 0; i < MR + 2 * R; i++)
 #pragma unroll
           for (int k = 0; k <= R; k++) {
             const double2 q = *reinterpret_cast<const double2*>(in + (r0 + i) * RS + c + 2 * k);
@@ -258,15 +270,15 @@ __global__ __launch_bounds__(256) void k_direct3d(DirectArgs a) {
               if (STAR && (int(e0 != R) + int(e1 != R) + int(e2 != R)) > 1) continue;
               const double gw = a.g[(e0 * U + e1) * U + e2];
 #pragma unroll
-              for (int i = 0; i < 4; i++)
+              for (int i = 0; i < MR; i++)
 #pragma unroll
                 for (int b = 0; b < 2; b++) acc[s][k][i][b] = __builtin_fma(gw, v[i + e1][b + e2], acc[s][k][i][b]);
             }
         }
         const int ke = (p + 1) % U;
-        double o[4][2];
+        double o[MR][2];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < MR; i++)
 #pragma unroll
           for (int b = 0; b < 2; b++) {
             o[i][b] = acc[s][ke][i][b];
@@ -274,14 +286,14 @@ __global__ __launch_bounds__(256) void k_direct3d(DirectArgs a) {
           }
         if (s + 1 < T) {
 #pragma unroll
-          for (int i = 0; i < 4; i++)
+          for (int i = 0; i < MR; i++)
             *reinterpret_cast<double2*>(&buf[s + 1][sl][(r0 + i) * RS + c]) = make_double2(o[i][0], o[i][1]);
         } else {
           const int64_t j = m - kShift;
           if (j >= 0 && j < planes) {
             double* pl = a.dst + (z0 + j) * n1 * n2;
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < MR; i++)
 #pragma unroll
               for (int b = 0; b < 2; b++)
                 if (str[i] && stc[b]) pl[(y0 + r0 + i) * n2 + x0 + c + b] = o[i][b];
@@ -299,7 +311,7 @@ void go2d(const DirectArgs& a, dim3 grid, hipStream_t s) {
 
 template <int R, int T, bool STAR>
 void go3d(const DirectArgs& a, dim3 grid, hipStream_t s) {
-  k_direct3d<R, T, STAR><<<grid, 256, 0, s>>>(a);
+  k_direct3d<R, T, STAR><<<grid, kDir3Threads, 0, s>>>(a);
 }
 
 template <int R, bool STAR, int... Ts>
@@ -328,7 +340,7 @@ int direct_max_depth(int dims, int radius, bool star) {
     return radius == 3 ? 2 : 0;
   }
   if (radius == 0) return 3;
-  return radius == 1 ? 2 : 0;
+  return radius == 1 ? (star ? 3 : 2) : 0;
 }
 
 hipError_t launch_direct(const DirectArgs& a, int dims, int radius, bool star, hipStream_t s) {
@@ -362,7 +374,7 @@ hipError_t launch_direct(const DirectArgs& a, int dims, int radius, bool star, h
   switch (radius) {
     case 0: return launch3d<0, false>(T, a, grid, s, std::make_integer_sequence<int, 3>{});
     case 1:
-      return star ? launch3d<1, true>(T, a, grid, s, std::make_integer_sequence<int, 2>{})
+      return star ? launch3d<1, true>(T, a, grid, s, std::make_integer_sequence<int, 3>{})
                   : launch3d<1, false>(T, a, grid, s, std::make_integer_sequence<int, 2>{});
     default: return hipErrorInvalidValue;
   }

@@ -61,7 +61,10 @@ constexpr int kDir1Pts = 8;                    // 1D: points per thread, in regi
 constexpr int kDir1Window = 256 * kDir1Pts;    // 1D: window per workgroup
 constexpr int kDir1MaxDepth = 64;              // 1D: largest fused depth of one launch
 constexpr int kDir2Cols = 512;                 // 2D: window columns per workgroup (2 per thread)
-constexpr int kDir3Rows = 64, kDir3Cols = 32;  // 3D: plane window (axis 1 x axis 2), 4 x 2 per thread
+constexpr int kDir3Threads = 512;              // 3D: threads per workgroup
+constexpr int kDir3PerThread = 2;              // 3D: points per thread along axis 1 (2 along axis 2)
+constexpr int kDir3Cols = 64;                  // 3D: plane window along the contiguous axis 2 (32 threads x 2)
+constexpr int kDir3Rows = kDir3Threads / (kDir3Cols / 2) * kDir3PerThread;  // along axis 1
 
 struct DirectArgs {
   const double* src;
