@@ -222,9 +222,9 @@ ffs_status choose_config(ffs_plan* pl) {
   if (p.algorithm == FFS_ALGO_DIRECT && p.precision != FFS_PREC_FP64)
     return fail(FFS_ERR_UNSUPPORTED, "the direct algorithm computes in FP64: use FFS_PREC_FP64");
   if (p.algorithm == FFS_ALGO_DIRECT) return choose_config_direct(pl);
-  // AUTO: direct in 2D/3D (measured on MI355X: Box2D25P 8192^2 601 vs 152 GStencils/s FP64 FFT, Box2D9P 1323
-  // vs 324, Box3D27P 512^3 289 vs 121), FFT in 1D (Heat-1D 2^20 x 64: 240-point PFA 1784 vs direct 1404).
-  if (p.algorithm == FFS_ALGO_AUTO && p.dims > 1 && ffsk::direct_max_depth(p.dims, r, star_support(pl)) > 0) {
+  // AUTO: direct wherever it is offered (MI355X, GStencils/s direct vs FP64 FFT: Box2D25P 8192^2 756 vs 152,
+  // Box3D27P 512^3 386 vs 121, 1D7P 2^20 1478 vs 1013, Heat-1D 2^26 5667 vs 4120).
+  if (p.algorithm == FFS_ALGO_AUTO && ffsk::direct_max_depth(p.dims, r, star_support(pl)) > 0) {
     pl->prob.precision = FFS_PREC_FP64;
     return choose_config_direct(pl);
   }

@@ -19,8 +19,8 @@
 //
 // 3D windows are 32 x 64 points per plane, 512 threads.
 //
-// 1D keeps a 2048-point window in registers (8 points per thread); each step exchanges the first 2r
-// values of every thread with its left neighbour through LDS (one barrier per step).
+// 1D keeps a window of kDir1Window points in registers (kDir1Pts per thread plus a redundant run of the
+// right neighbour's), exchanging through LDS once every kDir1Pts / 2r steps.
 #include <hip/hip_runtime.h>
 
 #include <utility>
@@ -36,46 +36,56 @@ __device__ __forceinline__ int64_t dwrap(int64_t i, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------------------------------
-// 1D: 256 threads x kDir1Pts points in registers.
+// 1D: 256 threads x kDir1Pts points in registers.  Each thread also holds the first H = 2r*K values of its
+// right neighbour, so K steps run between two exchanges through LDS: the neighbour's values are
+// recomputed redundantly (the held run shrinks by 2r per step) in exchange for K times fewer barriers.
 template <int R>
 __global__ __launch_bounds__(256) void k_direct1d(DirectArgs a) {
   constexpr int E = kDir1Pts;
   constexpr int W = kDir1Window;
-  constexpr int NE = R > 0 ? 2 * R : 1;
-  __shared__ double win[W + W / E];  // one pad double per thread's run of E points
-  __shared__ double edge[2][257][NE];
+  constexpr int K = R > 0 ? E / (2 * R) : E;  // steps per exchange
+  constexpr int H = 2 * R * K;                 // neighbour values held, <= E
+  constexpr int XS = H + 1;                    // exchange row stride (odd: no LDS bank conflicts)
+  static_assert(257 * XS <= W + W / E + 32, "exchange rows must fit the staging buffer");
+  __shared__ double win[W + W / E + 32];  // window staging (one pad per thread's run), then the exchange rows
   const int tid = threadIdx.x;
   const int64_t n = a.n0;
   const int64_t x0 = (int64_t)blockIdx.x * a.valid1;
   const int64_t base = x0 - (int64_t)a.steps * a.lo;
-  for (int i = tid; i < W; i += 256) win[i + i / E] = a.src[dwrap(base + i, n)];
-  if (tid < NE) {
-    edge[0][256][tid] = 0.0;
-    edge[1][256][tid] = 0.0;
+  if (base >= 0 && base + W <= n) {  // interior window: no wrap arithmetic
+    for (int i = tid; i < W; i += 256) win[i + i / E] = a.src[base + i];
+  } else {
+    for (int i = tid; i < W; i += 256) win[i + i / E] = a.src[dwrap(base + i, n)];
   }
+  for (int i = W + W / E + tid; i < W + W / E + 32; i += 256) win[i] = 0.0;
   __syncthreads();
-  double u[E];
+  double u[E + (H > 0 ? H : 1)];
 #pragma unroll
   for (int k = 0; k < E; k++) u[k] = win[tid * (E + 1) + k];
   double g[2 * R + 1];
 #pragma unroll
   for (int e = 0; e <= 2 * R; e++) g[e] = a.g[e];
-  for (int s = 0; s < a.steps; s++) {
-    const int b = s & 1;
-    double nb[NE];
-    if constexpr (R > 0) {
+  __syncthreads();  // the staging buffer becomes the exchange buffer
+  for (int s = 0; s < a.steps; s += K) {
+    if constexpr (H > 0) {
 #pragma unroll
-      for (int k = 0; k < 2 * R; k++) edge[b][tid][k] = u[k];
+      for (int k = 0; k < H; k++) win[tid * XS + k] = u[k];
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < 2 * R; k++) nb[k] = edge[b][tid + 1][k];
+      for (int k = 0; k < H; k++) u[E + k] = win[(tid + 1) * XS + k];  // row 256: stale, beyond the valid part
+      __syncthreads();
     }
+    const int ks = min(K, a.steps - s);
 #pragma unroll
-    for (int i = 0; i < E; i++) {  // ascending, in place: u[i + e] (e >= 0) is still the old value
-      double acc = 0.0;
+    for (int j = 1; j <= K; j++) {
+      if (j > ks) break;
 #pragma unroll
-      for (int e = 0; e <= 2 * R; e++) acc = __builtin_fma(g[e], i + e < E ? u[i + e] : nb[i + e - E], acc);
-      u[i] = acc;
+      for (int i = 0; i < E + H - 2 * R * j; i++) {  // ascending, in place: u[i + e] is still the old value
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e <= 2 * R; e++) acc = __builtin_fma(g[e], u[i + e], acc);
+        u[i] = acc;
+      }
     }
   }
   __syncthreads();

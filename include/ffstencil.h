@@ -72,10 +72,10 @@ typedef enum ffs_convention { FFS_CONV_REF = 0, FFS_CONV_CENTRED = 1 } ffs_conve
  *           blocking); precision must be FFS_PREC_FP64.  No kernel counterpart in the reference (its
  *           CPU check, src/2D/check_correct.hpp:64-100, is the semantics): a fallback and a baseline.
  *           Radius <= 3 in 1D and 2D, <= 1 in 3D.
- *   AUTO:   DIRECT in 2D and 3D where it is offered, else FFT.  On MI355X the FP64 vector and FP64 matrix
- *           peaks are equal, so the direct sum, with several times fewer FP64 operations per point-step
- *           than the transform path at these radii, is the faster one there; in 1D the 240-point
- *           prime-factor FFT tile is faster.  A DIRECT result is FP64 whatever precision was asked. */
+ *   AUTO:   DIRECT where it is offered, else FFT.  On MI355X the FP64 vector and FP64 matrix peaks are
+ *           equal, so the direct sum, with several times fewer FP64 operations per point-step than the
+ *           transform path at these radii, is the faster one (ffs_autotune with AUTO measures both).
+ *           A DIRECT result is FP64 whatever precision was asked. */
 typedef enum ffs_algorithm { FFS_ALGO_FFT = 0, FFS_ALGO_DIRECT = 1, FFS_ALGO_AUTO = 2 } ffs_algorithm;
 
 typedef struct ffs_problem {

@@ -146,9 +146,9 @@ def test_auto_and_precision_rules(ffs, oracle, torch_cuda):
     plan = ffs.Plan(3, (20, 20, 20), np.full(125, 1 / 125), 2, 2, algorithm="auto")
     assert plan.info["algorithm"] == 0
     plan.close()
-    # and keeps the FFT path in 1D
+    # and takes the direct path in 1D as well
     plan = ffs.Plan(1, (5000,), np.array([0.25, 0.5, 0.25]), 1, 8, algorithm="auto")
-    assert plan.info["algorithm"] == 0
+    assert plan.info["algorithm"] == 1
     plan.close()
 
 
