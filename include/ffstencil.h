@@ -88,7 +88,8 @@ typedef struct ffs_problem {
   int32_t convention;    /* ffs_convention */
   int32_t tile;          /* DFT length per axis; 0 = automatic (2D/3D: 16; 1D: 240 = 16x15 prime-factor) */
   int32_t fuse;          /* fused depth t; 0 = automatic */
-  int32_t use_graph;     /* 1 = replay the block loop from a hipGraph, 0 = plain launches */
+  int32_t use_graph;     /* 1 = replay the block loop from a hipGraph (chains of more than 4 launches;
+                            shorter chains launch directly, which is cheaper), 0 = plain launches */
   int32_t device;        /* HIP device ordinal */
   int32_t algorithm;     /* ffs_algorithm (default FFS_ALGO_FFT) */
   int32_t reserved[3];

@@ -111,8 +111,8 @@ def test_direct_zero_steps_and_graph_bitwise(ffs, oracle, torch_cuda):
     assert np.array_equal(got, x)
     d_in = torch.from_numpy(x).cuda()
     outs = []
-    for use_graph in (True, False):
-        plan = ffs.Plan(2, x.shape, w, r, 9, use_graph=use_graph, algorithm="direct")
+    for use_graph in (True, False):  # 41 steps at depth 4: an 11-launch chain, replayed from a graph
+        plan = ffs.Plan(2, x.shape, w, r, 41, use_graph=use_graph, algorithm="direct")
         d_out = torch.empty_like(d_in)
         plan.run(d_in, d_out)
         plan.run(d_in, d_out)
