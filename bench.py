@@ -4,9 +4,10 @@
 One bench "step" is one full pass of the hot path: T stencil steps over the whole synthetic grid
 (ffs_run: floor(T/t) fused blocks + tail, replayed from a hipGraph), input already resident in HBM.
 Default workload: Box2D25P, 8192 x 8192, T = 256, weights uniform[0,1) normalised to sum 1, grid
-uniform[0,1), both from SplitMix64 seeded 0x5EED0002.  `value` is the mixed-precision form (BF16
-split operands, FP32 accumulation, FP64 state and spectrum power); the plain FP64-MFMA form is timed
-beside it and both are checked against the FP64 direct sweep (the oracle) on a bounded sample.
+uniform[0,1), both from SplitMix64 seeded 0x5EED0002.  `value` is the fastest mixed-precision form
+(FP16 hi/lo split operands, FP32 accumulation, FP64 state and spectrum power); the BF16 3-term form,
+the plain FP64-MFMA form and the direct FP64 kernel are timed beside it, and all are checked against
+the FP64 direct sweep (the oracle) on a bounded sample.
 
 Printed: one JSON line (see the keys below).  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -27,12 +28,14 @@ PEAKS = {  # MI355X published dense peaks
     "bf16": ("mfma", 2.5e15, "TFLOP/s"),
     "f32": ("mfma", 157.3e12, "TFLOP/s"),
     "bf16x2": ("mfma", 2.5e15, "TFLOP/s"),
+    "f16": ("mfma", 2.5e15, "TFLOP/s"),
     "direct": ("valu", 78.6e12, "TFLOP/s"),  # FP64 vector peak
 }
 HBM_PEAK = 8.0e12
 
 WORKLOADS = {
-    "box2d25p": dict(shape="Box2D25P", sizes=(8192, 8192), T=256, seed=0x5EED0002, forms=("bf16", "fp64", "direct")),
+    "box2d25p": dict(shape="Box2D25P", sizes=(8192, 8192), T=256, seed=0x5EED0002,
+                     forms=("f16", "bf16", "fp64", "direct")),
     "box2d9p": dict(shape="Box2D9P", sizes=(8192, 8192), T=128, seed=0x5EED0001, forms=("fp64", "direct")),
     "heat1d": dict(shape="Heat-1D", sizes=(1048576,), T=64, seed=0x5EED0000, forms=("fp64", "direct")),
     "box3d27p": dict(shape="Box3D27P", sizes=(512, 512, 512), T=32, seed=0x5EED0003,
@@ -289,7 +292,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "bf16": "bf16x3-split/f32-acc (f64 state)", "bf16x2": "bf16x2-split/f32-acc (f64 state)",
-                  "f32": "f32 (f64 state)", "direct": "f64 (direct sum)"}[main_form],
+                  "f32": "f32 (f64 state)", "f16": "f16x2-split/f32-acc (f64 state)",
+                  "direct": "f64 (direct sum)"}[main_form],
         "data": "synthetic (SplitMix64 uniform[0,1) grid and weights)",
         "config": {
             "workload": f"{wl['shape']} {'x'.join(map(str, sizes))} T={T}",

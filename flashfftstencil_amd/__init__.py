@@ -34,7 +34,7 @@ STATUS_NAMES = {
     5: "FFS_ERR_NO_DEVICE",
     6: "FFS_ERR_INTERNAL",
 }
-PRECISIONS = {"fp64": 0, "bf16": 1, "mixed": 1, "f32": 2, "bf16x2": 3}
+PRECISIONS = {"fp64": 0, "bf16": 1, "mixed": 1, "f32": 2, "bf16x2": 3, "f16": 4}
 CONVENTIONS = {"ref": 0, "centred": 1, "centered": 1}
 ALGORITHMS = {"fft": 0, "direct": 1, "auto": 2}
 

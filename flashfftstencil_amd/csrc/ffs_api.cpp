@@ -80,6 +80,7 @@ struct ffs_plan {
   uint32_t* d_opsm = nullptr;  // mixed-form operand table
   double* d_H = nullptr;       // [main | tail], each 2*len doubles
   double dc_main = 1.0, dc_tail = 1.0;  // K(0)^t and K(0)^t_tail
+  int32_t scale_main = 0, scale_tail = 0;  // FP16 form: scale exponents of the main and tail depths
   double* d_taps = nullptr;
   double* d_scratch = nullptr;  // ping-pong grid
   int64_t npoints = 0;
@@ -117,7 +118,7 @@ ffs_status validate(const ffs_problem* p, int64_t* npoints) {
   if (p->radius < 0) return fail(FFS_ERR_INVALID_ARG, "radius must be >= 0 (got %d)", p->radius);
   if (p->steps < 0) return fail(FFS_ERR_INVALID_ARG, "steps must be >= 0 (got %lld)", (long long)p->steps);
   if (!p->weights) return fail(FFS_ERR_INVALID_ARG, "weights pointer is NULL");
-  if (p->precision < FFS_PREC_FP64 || p->precision > FFS_PREC_MIXED_BF16X2)
+  if (p->precision < FFS_PREC_FP64 || p->precision > FFS_PREC_MIXED_F16)
     return fail(FFS_ERR_INVALID_ARG, "unknown precision %d", p->precision);
   if (p->convention != FFS_CONV_REF && p->convention != FFS_CONV_CENTRED)
     return fail(FFS_ERR_INVALID_ARG, "unknown convention %d", p->convention);
@@ -230,6 +231,8 @@ ffs_status choose_config(ffs_plan* pl) {
     pl->prob.precision = FFS_PREC_FP64;
     return choose_config_direct(pl);
   }
+  if (p.precision == FFS_PREC_MIXED_F16 && p.dims != 2)
+    return fail(FFS_ERR_UNSUPPORTED, "the FP16 mixed form is offered for 2D grids (32-point tiles) only");
   if (p.dims == 1) {
     if (p.tile != 0 && p.tile != 240)
       return fail(FFS_ERR_UNSUPPORTED, "1D tile must be 240 (15x16 prime-factor window), got %d", p.tile);
@@ -239,9 +242,11 @@ ffs_status choose_config(ffs_plan* pl) {
     pl->L[1] = 16;
   } else if (p.dims == 2) {
     const bool bf = p.precision == FFS_PREC_MIXED_BF16 || p.precision == FFS_PREC_MIXED_BF16X2;
-    const int P = p.tile != 0 ? p.tile : (bf ? 32 : 16);
-    if (!(P == 16 || (P == 32 && bf)))
-      return fail(FFS_ERR_UNSUPPORTED, "2D tile %d not available for this precision (16; 32 with the BF16 forms)", P);
+    const bool f16 = p.precision == FFS_PREC_MIXED_F16;
+    const int P = p.tile != 0 ? p.tile : (bf || f16 ? 32 : 16);
+    if (!((P == 16 && !f16) || (P == 32 && (bf || f16))))
+      return fail(FFS_ERR_UNSUPPORTED,
+                  "2D tile %d not available for this precision (16 or 32 with the BF16 forms, 32 with FP16, else 16)", P);
     pl->kind = P == 32 ? ffsk::KIND_2D32 : ffsk::KIND_2D16;
     pl->P[0] = pl->P[1] = P;
     pl->L[0] = pl->L[1] = P;
@@ -268,7 +273,8 @@ ffs_status choose_config(ffs_plan* pl) {
     // at 2.4 GHz).
     const int dd = p.dims == 1 ? 1 : p.dims;
     double compute = p.precision == FFS_PREC_FP64 ? 4700.0 : (p.precision == FFS_PREC_MIXED_F32 ? 2600.0 : 2500.0);
-    if (pl->kind == ffsk::KIND_2D32) compute = p.precision == FFS_PREC_MIXED_BF16 ? 12000.0 : 9000.0;
+    if (pl->kind == ffsk::KIND_2D32)
+      compute = p.precision == FFS_PREC_MIXED_BF16 ? 12000.0 : (p.precision == FFS_PREC_MIXED_F16 ? 3000.0 : 9000.0);
     const double hbm_per_point = 2.0 * 16.0 / 3.26;
     double best = -1;
     tbest = 1;
@@ -347,6 +353,7 @@ ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, boo
   a.h1 = h[1];
   a.h2 = h[2];
   a.precision = p.precision;
+  a.scale_exp = tail ? pl->scale_tail : pl->scale_main;
   return a;
 }
 
@@ -582,7 +589,7 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
   chk(hipMemcpy(pl->d_taps, pl->weights.data(), sizeof(double) * ntaps, hipMemcpyHostToDevice));
   const int layout = p->precision == FFS_PREC_FP64 ? 0 : 1;
   if (pl->kind == ffsk::KIND_2D32)
-    chk(ffsk::gen_operands32(pl->d_opsm, nullptr));
+    chk(ffsk::gen_operands32(pl->d_opsm, p->precision == FFS_PREC_MIXED_F16, nullptr));
   else
     chk(ffsk::gen_operands(pl->d_ops, pl->d_opsm, pl->L, nullptr));
   chk(ffsk::gen_spectrum(pl->d_H, pl->kind, layout, pl->L, pl->d_taps, K, p->dims, p->convention, p->radius, pl->t,
@@ -600,6 +607,35 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
     };
     pl->dc_main = ipow(sw, pl->t);
     pl->dc_tail = ipow(sw, pl->t_tail > 0 ? pl->t_tail : pl->t);
+  }
+  if (p->precision == FFS_PREC_MIXED_F16) {
+    // largest |K(k)| over the 32 x 32 frequencies (direct DFT of the taps; |K| does not depend on the
+    // convention), then scale_exp = floor(10 - log2 max(1, |K|^t)) for each depth
+    double kmax = 0.0;
+    const int P = 32;
+    for (int k0 = 0; k0 < P; k0++)
+      for (int k1 = 0; k1 < P; k1++) {
+        double re = 0.0, im = 0.0;
+        for (int a0 = 0; a0 < K; a0++)
+          for (int a1 = 0; a1 < K; a1++) {
+            const double ang = -2.0 * M_PI * (double)((a0 * k0 + a1 * k1) % P) / P;
+            re += pl->weights[a0 * K + a1] * std::cos(ang);
+            im += pl->weights[a0 * K + a1] * std::sin(ang);
+          }
+        kmax = std::max(kmax, std::sqrt(re * re + im * im));
+      }
+    auto expo = [&](int64_t t) {
+      const double lg = kmax > 1.0 ? (double)t * std::log2(kmax) : 0.0;
+      return std::floor(10.0 - lg);
+    };
+    const double em = expo(pl->t), et = expo(pl->t_tail > 0 ? pl->t_tail : pl->t);
+    if (em < -100.0 || et < -100.0) {
+      release(pl);
+      return fail(FFS_ERR_UNSUPPORTED, "FP16 form: |K|^t = 2^%.0f is beyond the FP16 range; use another form",
+                  10.0 - std::min(em, et));
+    }
+    pl->scale_main = (int32_t)em;
+    pl->scale_tail = (int32_t)et;
   }
   chk(hipDeviceSynchronize());
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&pl->cap_stream, hipStreamNonBlocking);
@@ -691,7 +727,8 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   if (p.precision == FFS_PREC_MIXED_F32) per_cplx = 4.0 * gemm;
   double per_pair = 2.0 * dd * per_cplx * std::pow(16.0, dd - 2);
   if (pl->kind == ffsk::KIND_2D32)  // 4 output blocks x 4 real GEMMs x (6 or 4) K=32 MFMAs, 4 stages
-    per_pair = 4.0 * 4.0 * 4.0 * (p.precision == FFS_PREC_MIXED_BF16 ? 6.0 : 4.0) * 2.0 * 16 * 16 * 32;
+    per_pair = 4.0 * 4.0 * 4.0 * (p.precision == FFS_PREC_MIXED_BF16 ? 6.0 : (p.precision == FFS_PREC_MIXED_F16 ? 3.0 : 4.0)) *
+               2.0 * 16 * 16 * 32;
   info->flops_per_block = per_pair * (double)pl->pairs;
   info->bytes_per_block = 16.0 * (double)pl->npoints;
   return FFS_OK;
@@ -842,6 +879,7 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
     if (p->dims == 1) tiles = {240};
     else if (p->dims == 3) tiles = {16};
     else if (p->precision == FFS_PREC_MIXED_BF16 || p->precision == FFS_PREC_MIXED_BF16X2) tiles = {16, 32};
+    else if (p->precision == FFS_PREC_MIXED_F16) tiles = {32};
     else tiles = {16};
     for (int tile : tiles) {
       const int vmin = p->dims == 1 ? 16 : 2;
@@ -936,8 +974,8 @@ ffs_status ffs_batch_create(const ffs_problem* problems, int32_t count, ffs_batc
       return fail(FFS_ERR_UNSUPPORTED, "batch problem %d: batches run the FFT algorithm only", i);
     if (problems[i].precision != prec)
       return fail(FFS_ERR_INVALID_ARG, "batch problem %d: all problems must share one precision", i);
-    if (prec != FFS_PREC_MIXED_BF16 && prec != FFS_PREC_MIXED_BF16X2)
-      return fail(FFS_ERR_UNSUPPORTED, "batches run the BF16 mixed forms (32-point tiles)");
+    if (prec != FFS_PREC_MIXED_BF16 && prec != FFS_PREC_MIXED_BF16X2 && prec != FFS_PREC_MIXED_F16)
+      return fail(FFS_ERR_UNSUPPORTED, "batches run the BF16 and FP16 mixed forms (32-point tiles)");
     if (problems[i].tile != 0 && problems[i].tile != 32)
       return fail(FFS_ERR_UNSUPPORTED, "batch problem %d: tile must be 32", i);
     int64_t np = 0;

@@ -36,7 +36,8 @@ inline int64_t spectrum_len(int kind) {  // complex values per spectrum (one dep
   return kind == KIND_3D16 ? 16 * 16 * 16 : (kind == KIND_2D32 ? 32 * 32 : 16 * 16);
 }
 
-// 32-point BF16 B operands (KIND_2D32): [arr: R, I, NI][column block Kb][term h, m, l][word 0..3][lane];
+// 32-point B operands (KIND_2D32): [arr: R, I, NI][column block Kb][term][word 0..3][lane]; terms h, m, l of
+// the 3-term BF16 split, or h, l (and a zero term) of the 2-term FP16 split for the FP16 form;
 // element e of lane (j + 16q) is row 16 (e >> 2) + 4q + (e & 3), column 16 Kb + j of that term.
 constexpr int kOps32Words = 3 * 2 * 3 * 4 * kLanes;  // 4608
 
@@ -52,7 +53,7 @@ struct TileArgs {
   int32_t V0, V1, V2;       // valid widths
   int32_t h0, h1, h2;       // window start offset (valid region starts at h in the window)
   int32_t precision;
-  int32_t pad_;
+  int32_t scale_exp;  // FP16 form: the tile pair is scaled so that its largest fluctuation is below 2^scale_exp
   double dc_gain;  // K(0)^t: gain of the constant mode (mixed forms add mean * gain back in FP64)
 };
 
@@ -84,7 +85,7 @@ hipError_t launch_direct(const DirectArgs& a, int dims, int radius, bool star, h
 
 // Launch helpers implemented in ffs_kernels.hip.
 hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hipStream_t s);
-hipError_t gen_operands32(uint32_t* ops32, hipStream_t s);
+hipError_t gen_operands32(uint32_t* ops32, bool f16, hipStream_t s);
 // layout 0: FP64 accumulator order (row freq q + 4v); layout 1: FP32 accumulator order (row freq 4q + v)
 hipError_t gen_spectrum(double* H, int kind, int layout, const int32_t* axis_len, const double* d_taps,
                         int32_t ntaps_per_axis, int32_t dims, int32_t convention, int32_t radius, int64_t t,

@@ -323,7 +323,7 @@ typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 #define FFS_MFMA_BF16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16((a), (b), (c), 0, 0, 0)
 #define FFS_MFMA_F32(a, b, c) __builtin_amdgcn_mfma_f32_16x16x4f32((a), (b), (c), 0, 0, 0)
 
-enum { PREC_BF16 = 1, PREC_F32 = 2, PREC_BF16X2 = 3 };
+enum { PREC_BF16 = 1, PREC_F32 = 2, PREC_BF16X2 = 3, PREC_F16 = 4 };
 
 template <int NTERMS>
 struct OpsBF16 {
@@ -709,7 +709,7 @@ __device__ __forceinline__ void gather32(const TileArgs& a, const PairCoord& pc,
 }
 
 __device__ __forceinline__ void store32(const TileArgs& a, const PairCoord& pc, int q, int j, const Blk4& yr,
-                                        const Blk4& yi, double addA, double addB) {
+                                        const Blk4& yi, double addA, double addB, double scale = 1.0) {
 #pragma unroll
   for (int C = 0; C < 2; C++) {
     const int wc = 16 * C + j;
@@ -725,8 +725,8 @@ __device__ __forceinline__ void store32(const TileArgs& a, const PairCoord& pc, 
         const int64_t y = pc.r0 + row - a.h0;
         if (row >= a.h0 && row < a.h0 + a.V0 && y < a.n0) {
           double* drow = a.dst + y * a.n1;
-          if (okA) drow[cA] = (double)yr.b[2 * R + C][v] + addA;
-          if (okB) drow[cB] = (double)yi.b[2 * R + C][v] + addB;
+          if (okA) drow[cA] = (double)yr.b[2 * R + C][v] * scale + addA;
+          if (okB) drow[cB] = (double)yi.b[2 * R + C][v] * scale + addB;
         }
       }
   }
@@ -770,6 +770,131 @@ __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const
   stage32<true, NTERMS>(zr, zi, sF, lane, orr, oi);
   stage32<true, NTERMS>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
   store32(a, pc, q, j, zr, zi, cA * a.dc_gain, cB * a.dc_gain);
+}
+
+// ---------------------------------------------------------------------------------------------
+// 2D 32x32 tiles, mixed FP16 form.  Each A operand is split into two FP16 terms (x = h + l, 11 + 11
+// significand bits, residual <= 2^-23 |x|) and the DFT operands likewise; a real 16x16 output block keeps
+// the three products hh, hl, lh (3 MFMAs instead of the BF16 form's 6; the dropped ll is ~2^-22 of the
+// result), so the MFMA work and the split work are both about half.  FP16's range is what BF16 does not
+// need: the tile pair is scaled by a power of two (exact) so that its largest fluctuation lies below
+// 2^scale_exp, with scale_exp = 10 - log2 max(1, max|K^t|) chosen on the host; then every split operand
+// stays below 32 sqrt(2) 2^10 < 65504 through the four stages, and values too small for FP16 normals
+// sit ~2^-34 below the tile's largest and do not matter.  The store undoes the scale (exact) in FP64.
+// ---------------------------------------------------------------------------------------------
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+#define FFS_MFMA_F16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_f16((a), (b), (c), 0, 0, 0)
+
+__device__ __forceinline__ h8 as_h8(u4 w) { return __builtin_bit_cast(h8, w); }
+// Two FP32 values -> one word of two FP16 (round to nearest even, one v_cvt_pk_f16_f32).
+__device__ __forceinline__ uint32_t pk_f16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{a, b}, h2));
+}
+__device__ __forceinline__ float f16_lo(uint32_t w) { return (float)__builtin_bit_cast(h2, w)[0]; }
+__device__ __forceinline__ float f16_hi(uint32_t w) { return (float)__builtin_bit_cast(h2, w)[1]; }
+
+// Same element order as split8: word k of the operand holds elements (2k, 2k+1) of (x0 | x1).
+__device__ __forceinline__ void split8_f16(const f4& x0, const f4& x1, h8& h, h8& l) {
+  u4 hw, lw;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const f4& x = k < 2 ? x0 : x1;
+    const int e = 2 * (k & 1);
+    hw[k] = pk_f16(x[e], x[e + 1]);
+    lw[k] = pk_f16(x[e] - f16_lo(hw[k]), x[e + 1] - f16_hi(hw[k]));
+  }
+  h = as_h8(hw);
+  l = as_h8(lw);
+}
+
+template <bool INV>
+__device__ __forceinline__ void stage32_f16(const Blk4& zr, const Blk4& zi, const u4* __restrict__ sF, int lane,
+                                            Blk4& outr, Blk4& outi) {
+#pragma unroll
+  for (int I = 0; I < 2; I++) {
+    h8 rh, rl, ih, il;
+    split8_f16(zr.b[I], zr.b[2 + I], rh, rl);
+    split8_f16(zi.b[I], zi.b[2 + I], ih, il);
+#pragma unroll
+    for (int Kb = 0; Kb < 2; Kb++) {
+      h8 Fr[2], Fi[2], Fn[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        Fr[t] = as_h8(sF[((0 * 2 + Kb) * 3 + t) * kLanes + lane]);
+        Fi[t] = as_h8(sF[((1 * 2 + Kb) * 3 + t) * kLanes + lane]);
+        Fn[t] = as_h8(sF[((2 * 2 + Kb) * 3 + t) * kLanes + lane]);
+      }
+      // forward: Re = zr Fr + zi (-Fi), Im = zr Fi + zi Fr;  inverse: Re = zr Fr + zi Fi, Im = zr (-Fi) + zi Fr
+      const h8* Gi = INV ? Fn : Fi;  // multiplies zr into Im
+      const h8* Gr = INV ? Fi : Fn;  // multiplies zi into Re
+      f4 re = {0.f, 0.f, 0.f, 0.f}, iv = re;
+      re = FFS_MFMA_F16(rh, Fr[1], re);  // the cross products first, the two dominant ones last
+      re = FFS_MFMA_F16(rl, Fr[0], re);
+      iv = FFS_MFMA_F16(rh, Gi[1], iv);
+      iv = FFS_MFMA_F16(rl, Gi[0], iv);
+      re = FFS_MFMA_F16(ih, Gr[1], re);
+      re = FFS_MFMA_F16(il, Gr[0], re);
+      iv = FFS_MFMA_F16(ih, Fr[1], iv);
+      iv = FFS_MFMA_F16(il, Fr[0], iv);
+      re = FFS_MFMA_F16(rh, Fr[0], re);
+      iv = FFS_MFMA_F16(rh, Gi[0], iv);
+      re = FFS_MFMA_F16(ih, Gr[0], re);
+      iv = FFS_MFMA_F16(ih, Fr[0], iv);
+      outr.b[2 * I + Kb] = re;
+      outi.b[2 * I + Kb] = iv;
+    }
+  }
+}
+
+template <bool H_FROM_LDS>
+__device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, const u4* __restrict__ sF,
+                                                const f2* __restrict__ sH, int lane, int q, int j) {
+  const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
+  double xa[16], xb[16];
+  gather32(a, pc, q, j, xa, xb);
+  const double cA = lane_value(xa[12], 0), cB = lane_value(xb[12], 0);
+  Blk4 zr, zi, orr, oi;
+  float m = 0.f;
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      zr.b[b][r] = (float)(xa[b * 4 + r] - cA);
+      zi.b[b][r] = (float)(xb[b * 4 + r] - cB);
+      m = fmaxf(m, fmaxf(fabsf(zr.b[b][r]), fabsf(zi.b[b][r])));
+    }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  int k;
+  (void)frexpf(m, &k);  // m < 2^k
+  const int e = __builtin_amdgcn_readfirstlane(min(max(a.scale_exp - k, -120), 120));
+  const float sc = ldexpf(1.0f, e);
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      zr.b[b][r] *= sc;
+      zi.b[b][r] *= sc;
+    }
+  stage32_f16<false>(zr, zi, sF, lane, orr, oi);
+  stage32_f16<false>(orr, oi, sF, lane, zr, zi);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int ee = (b * 4 + v) * kLanes + lane;
+      const float hr = H_FROM_LDS ? sH[ee][0] : (float)a.H[ee];
+      const float hi = H_FROM_LDS ? sH[ee][1] : (float)a.H[1024 + ee];
+      float yr = zr.b[b][v], yi = zi.b[b][v];
+      cmul(yr, yi, hr, hi);
+      zr.b[b][v] = yr;
+      zi.b[b][v] = yi;
+    }
+  stage32_f16<true>(zr, zi, sF, lane, orr, oi);
+  stage32_f16<true>(orr, oi, sF, lane, zr, zi);
+  store32(a, pc, q, j, zr, zi, cA * a.dc_gain, cB * a.dc_gain, ldexp(1.0, -e));
 }
 
 __device__ __forceinline__ void stage_ops32(u4* sF, const uint32_t* __restrict__ opsm) {
@@ -825,6 +950,41 @@ __global__ __launch_bounds__(256, 3) void k_batch32_bf16(const TileArgs* __restr
   }
 }
 
+// The FP16 form of the two kernels above (same structure, FP16 operand pairs).
+__global__ __launch_bounds__(256, 3) void k_tile32_f16(TileArgs a) {
+  __shared__ u4 sF[3 * 2 * 3 * kLanes];
+  __shared__ f2 sH[4 * 4 * kLanes];
+  stage_ops32(sF, a.opsm);
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) sH[i] = f2{(float)a.H[i], (float)a.H[1024 + i]};
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, j = lane & 15;
+  const int64_t W = (int64_t)gridDim.x * 4;
+  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair_f16<true>(a, tp, sF, sH, lane, q, j);
+}
+
+__global__ __launch_bounds__(256, 3) void k_batch32_f16(const TileArgs* __restrict__ items,
+                                                        const int64_t* __restrict__ prefix, int32_t n) {
+  __shared__ u4 sF[3 * 2 * 3 * kLanes];
+  stage_ops32(sF, items[0].opsm);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, j = lane & 15;
+  const int64_t W = (int64_t)gridDim.x * 4;
+  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t total = prefix[n];
+  for (int64_t idx = gw; idx < total; idx += W) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (prefix[mid] <= idx) lo = mid; else hi = mid;
+    }
+    const TileArgs a = items[lo];
+    tile32_pair_f16<false>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+  }
+}
+
 hipError_t launch_batch(const TileArgs* d_items, const int64_t* d_prefix, int32_t n, int32_t precision, hipStream_t s,
                         int num_cus) {
   const dim3 g((unsigned)num_cus * 3), b(256);
@@ -832,6 +992,8 @@ hipError_t launch_batch(const TileArgs* d_items, const int64_t* d_prefix, int32_
     hipLaunchKernelGGL(k_batch32_bf16<3>, g, b, 0, s, d_items, d_prefix, n);
   else if (precision == PREC_BF16X2)
     hipLaunchKernelGGL(k_batch32_bf16<2>, g, b, 0, s, d_items, d_prefix, n);
+  else if (precision == PREC_F16)
+    hipLaunchKernelGGL(k_batch32_f16, g, b, 0, s, d_items, d_prefix, n);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -1133,8 +1295,9 @@ __global__ void k_gen_operands_mixed(uint32_t* t, int32_t L0, int32_t L1, int32_
   }
 }
 
-// 32-point BF16 B operands for KIND_2D32 (layout in ffs_internal.h); one thread per (arr, Kb, term, lane).
-__global__ void k_gen_operands32(uint32_t* t) {
+// 32-point B operands for KIND_2D32 (layout in ffs_internal.h); one thread per (arr, Kb, term, lane).
+// BF16 forms: terms h, m, l of the 3-term split.  FP16 form: terms h, l of the 2-term split (term 2 = 0).
+__global__ void k_gen_operands32(uint32_t* t, int32_t f16) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= 3 * 2 * 3 * kLanes) return;
   const int lane = idx % kLanes, term = (idx / kLanes) % 3, Kb = (idx / (3 * kLanes)) % 2, arr = idx / (6 * kLanes);
@@ -1146,16 +1309,23 @@ __global__ void k_gen_operands32(uint32_t* t) {
       double fr, fi;
       dft_entry(16 * (e >> 2) + 4 * q + (e & 3), 16 * Kb + j, 32, fr, fi);
       const double v = arr == 0 ? fr : (arr == 1 ? fi : -fi);
-      __bf16 h, m, l;
-      split3_bf16(v, h, m, l);
-      const __bf16 pick = term == 0 ? h : (term == 1 ? m : l);
-      word |= (uint32_t)__builtin_bit_cast(uint16_t, pick) << (16 * half);
+      uint16_t bits;
+      if (f16) {
+        const _Float16 h = (_Float16)(float)v;
+        const _Float16 l = (_Float16)(float)(v - (double)h);
+        bits = __builtin_bit_cast(uint16_t, term == 0 ? h : (term == 1 ? l : (_Float16)0.0f));
+      } else {
+        __bf16 h, m, l;
+        split3_bf16(v, h, m, l);
+        bits = __builtin_bit_cast(uint16_t, term == 0 ? h : (term == 1 ? m : l));
+      }
+      word |= (uint32_t)bits << (16 * half);
     }
     t[((((arr * 2 + Kb) * 3 + term) * 4) + w) * kLanes + lane] = word;
   }
 }
 
-// K(k) = sum_taps w[a]// K(k) = sum_taps w[a]// K(k) = sum_taps w[a] exp(-2 pi i sum_ax (d_ax(a) k_ax mod L_ax)/L_ax),  d = a (ref) or r - a (centred);
+// K(k) = sum_taps w[a] exp(-2 pi i sum_ax (d_ax(a) k_ax mod L_ax)/L_ax),  d = a (ref) or r - a (centred);
 // H = K^t / prod(L) by repeated squaring in complex FP64.  `kind` fixes which frequency sits at
 // entry e of the family layout.
 __global__ void k_gen_spectrum(double* H, int kind, int layout, int32_t L0, int32_t L1, int32_t L2,
@@ -1254,8 +1424,8 @@ hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hi
   return hipGetLastError();
 }
 
-hipError_t gen_operands32(uint32_t* ops32, hipStream_t s) {
-  hipLaunchKernelGGL(k_gen_operands32, dim3((3 * 2 * 3 * kLanes + 255) / 256), dim3(256), 0, s, ops32);
+hipError_t gen_operands32(uint32_t* ops32, bool f16, hipStream_t s) {
+  hipLaunchKernelGGL(k_gen_operands32, dim3((3 * 2 * 3 * kLanes + 255) / 256), dim3(256), 0, s, ops32, (int32_t)f16);
   return hipGetLastError();
 }
 
@@ -1312,6 +1482,8 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
       hipLaunchKernelGGL(k_tile32_bf16<3>, g, b, 0, s, a);
     else if (a.precision == PREC_BF16X2)
       hipLaunchKernelGGL(k_tile32_bf16<2>, g, b, 0, s, a);
+    else if (a.precision == PREC_F16)
+      hipLaunchKernelGGL(k_tile32_f16, g, b, 0, s, a);
     else
       return hipErrorInvalidValue;
   } else {

@@ -48,7 +48,8 @@ static void print_help() {
       "  --help               print this text\n"
       "  --custom FILE|-      read (2r+1)^%d stencil weights (row-major, ref convention) from FILE or stdin;\n"
       "                       r is the shape's radius\n"
-      "  --precision P        fp64 (default) | bf16 (mixed: BF16 split operands) | f32 (mixed: FP32 operands)\n"
+      "  --precision P        fp64 (default) | bf16 (mixed: BF16 split operands) | f16 (mixed: FP16 split\n"
+      "                       operands, 2D) | f32 (mixed: FP32 operands)\n"
       "  --convention C       ref (default, out[p] = sum_a w[a] in[p-a]) | centred (out[p] = sum_o w[o+r] in[p+o])\n"
       "  --algorithm A        fft (default: tiled FFT stencil) | direct (FP64 stencil sum, temporal blocking) |\n"
       "                       auto (direct where offered)\n"
@@ -185,7 +186,11 @@ int main(int argc, char** argv) {
   for (int d = 0; d < dims; d++) p.sizes[d] = sizes[d];
   p.weights = w.data();
   p.steps = T;
-  p.precision = prec == "fp64" ? FFS_PREC_FP64 : prec == "bf16" ? FFS_PREC_MIXED_BF16 : prec == "f32" ? FFS_PREC_MIXED_F32 : -1;
+  p.precision = prec == "fp64"   ? FFS_PREC_FP64
+                : prec == "bf16" ? FFS_PREC_MIXED_BF16
+                : prec == "f16"  ? FFS_PREC_MIXED_F16
+                : prec == "f32"  ? FFS_PREC_MIXED_F32
+                                 : -1;
   p.convention = conv == "ref" ? FFS_CONV_REF : conv == "centred" || conv == "centered" ? FFS_CONV_CENTRED : -1;
   p.algorithm = algo == "fft" ? FFS_ALGO_FFT : algo == "direct" ? FFS_ALGO_DIRECT : algo == "auto" ? FFS_ALGO_AUTO : -1;
   p.fuse = fuse;

@@ -55,7 +55,9 @@ typedef enum ffs_precision {
   FFS_PREC_FP64 = 0,       /* FP64 MFMA (v_mfma_f64_16x16x4f64) contractions: the drop-in form */
   FFS_PREC_MIXED_BF16 = 1,  /* 3-term BF16 split operands, FP32 accumulation (v_mfma_f32_16x16x32_bf16) */
   FFS_PREC_MIXED_F32 = 2,   /* FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4f32) */
-  FFS_PREC_MIXED_BF16X2 = 3 /* 2-term BF16 data split (16 bits): faster, meets 1e-6 only on smoothed fields */
+  FFS_PREC_MIXED_BF16X2 = 3, /* 2-term BF16 data split (16 bits): faster, meets 1e-6 only on smoothed fields */
+  FFS_PREC_MIXED_F16 = 4     /* 2-term FP16 split operands (22 bits) with a per-tile power-of-2 scale, FP32
+                                accumulation (v_mfma_f32_16x16x32_f16), 3 products per block; 2D only */
 } ffs_precision;
 
 /* Index convention of one step.

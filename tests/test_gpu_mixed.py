@@ -22,7 +22,7 @@ def _run(ffs, torch, x, radius, w, steps, **kw):
     return d_out.cpu().numpy(), info
 
 
-@pytest.mark.parametrize("form", ["bf16", "f32"])
+@pytest.mark.parametrize("form", ["bf16", "f32", "f16"])
 @pytest.mark.parametrize("shape", ["Heat-2D", "Box2D9P", "Star2D9P", "Box2D25P"])
 @pytest.mark.parametrize("size", [(37, 53), (16, 16), (64, 40), (3, 5)])
 @pytest.mark.parametrize("steps", [1, 3, 8])
@@ -35,7 +35,7 @@ def test_2d_mixed_named(ffs, oracle, torch_cuda, form, shape, size, steps):
     assert rel <= TOL_MIXED, (form, shape, size, steps, info, rel)
 
 
-@pytest.mark.parametrize("form", ["bf16", "f32", "bf16x2"])
+@pytest.mark.parametrize("form", ["bf16", "f32", "bf16x2", "f16"])
 def test_2d_mixed_long_horizon(ffs, oracle, torch_cuda, form):
     _, r, w, _ = ffs.named_stencil("Box2D25P", seed=0x5EED0002)
     x = oracle.uniform(0x5EED0003, (256, 256))
@@ -46,7 +46,7 @@ def test_2d_mixed_long_horizon(ffs, oracle, torch_cuda, form):
     assert rel <= TOL_MIXED, (form, info, rel, l2)
 
 
-@pytest.mark.parametrize("form", ["bf16", "f32"])
+@pytest.mark.parametrize("form", ["bf16", "f32", "f16"])
 @pytest.mark.parametrize("convention", ["ref", "centred"])
 @pytest.mark.parametrize("fuse", [1, 2, 3])
 def test_2d_mixed_fuse_convention(ffs, oracle, torch_cuda, form, convention, fuse):
@@ -59,7 +59,7 @@ def test_2d_mixed_fuse_convention(ffs, oracle, torch_cuda, form, convention, fus
     assert oracle.errors(got, ref)[1] <= TOL_MIXED
 
 
-@pytest.mark.parametrize("form", ["bf16", "f32"])
+@pytest.mark.parametrize("form", ["bf16", "f32", "f16"])
 def test_2d_mixed_gain_not_one(ffs, oracle, torch_cuda, form):
     # weights summing to 1.3: the constant mode is carried in FP64 with gain (sum w)^t
     rng = np.random.default_rng(9)
@@ -149,3 +149,31 @@ def test_3d_mixed_sinusoid_field(ffs, oracle, torch_cuda, form):
     # The BF16 form (K = 32 per rounding) meets the mixed-form bar of 1e-6.
     tol = TOL_MIXED if form == "bf16" else 2e-6
     assert rel <= tol, (info, rel, l2)
+
+
+@pytest.mark.parametrize("scale", [1e-30, 1e-6, 1.0, 1e6, 1e30])
+def test_2d_f16_scale_invariance(ffs, oracle, torch_cuda, scale):
+    # the FP16 form scales every tile pair to its own range: grids of any magnitude meet the same bar
+    _, r, w, _ = ffs.named_stencil("Box2D25P", seed=4)
+    x = oracle.uniform(12, (96, 80)) * scale
+    got, info = _run(ffs, torch_cuda, x, r, w, 7, precision="f16")
+    ref = oracle.sweep(x, r, w, 7)
+    rel = oracle.errors(got, ref)[1]
+    assert rel <= TOL_MIXED, (scale, info, rel)
+
+
+def test_2d_f16_unnormalised_gain(ffs, oracle, torch_cuda):
+    # |K| > 1 at some frequencies (signed weights): the host lowers the scale exponent by log2 |K|^t
+    rng = np.random.default_rng(3)
+    w = rng.standard_normal(25) * 0.3
+    x = oracle.uniform(5, (70, 90))
+    got, info = _run(ffs, torch_cuda, x, 2, w, 6, precision="f16")
+    ref = oracle.sweep(x, 2, w, 6)
+    assert oracle.errors(got, ref)[1] <= TOL_MIXED, info
+
+
+def test_f16_form_is_2d_only(ffs, torch_cuda):
+    with pytest.raises(ffs.FFSError, match="UNSUPPORTED"):
+        ffs.Plan(1, (100,), np.array([0.25, 0.5, 0.25]), 1, 4, precision="f16")
+    with pytest.raises(ffs.FFSError, match="UNSUPPORTED"):
+        ffs.Plan(2, (64, 64), np.full(9, 1 / 9), 1, 4, precision="f16", tile=16)
