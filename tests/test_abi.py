@@ -40,8 +40,8 @@ def test_library_exports_every_declared_symbol(ffs):
 
 
 def test_struct_layouts_match_header(ffs):
-    # ffs_problem: 2*i32, 3*i64, ptr, i64, 6*i32, 4*i32 reserved
-    assert ctypes.sizeof(ffs.Problem) == 4 * 2 + 8 * 3 + 8 + 8 + 4 * 6 + 4 * 4
+    # ffs_problem: 2*i32, 3*i64, ptr, i64, 7*i32 (algorithm last), 3*i32 reserved
+    assert ctypes.sizeof(ffs.Problem) == 4 * 2 + 8 * 3 + 8 + 8 + 4 * 7 + 4 * 3
     lib = ffs.load_library()
     p = ffs.Problem()
     assert lib.ffs_problem_init(ctypes.byref(p)) == 0
@@ -122,7 +122,8 @@ def _status(ffs, **kw):
 @pytest.mark.parametrize("kw,needle", [
     (dict(dims=0), "dims"), (dict(dims=4), "dims"), (dict(radius=-1), "radius"), (dict(steps=-3), "steps"),
     (dict(sizes=[0, 5]), "size"), (dict(weights=None), "weights"), (dict(precision=9), "precision"),
-    (dict(convention=5), "convention"), (dict(fuse=-1), "fuse"),
+    (dict(convention=5), "convention"), (dict(fuse=-1), "fuse"), (dict(algorithm=3), "algorithm"),
+    (dict(algorithm=-1), "algorithm"),
     (dict(weights=np.array([1.0, np.nan] + [0.0] * 7)), "finite"),
 ])
 def test_plan_create_rejects_invalid(ffs, kw, needle):
@@ -134,6 +135,11 @@ def test_plan_create_rejects_invalid(ffs, kw, needle):
 @pytest.mark.parametrize("kw,needle", [
     (dict(tile=24), "tile"), (dict(tile=32, precision=0), "tile"), (dict(fuse=8), "fuse"),
     (dict(radius=8, weights=np.full(17 * 17, 1 / 289.0)), "radius"),
+    (dict(precision=4, tile=16), "tile"), (dict(precision=4, dims=1, sizes=[640]), "FP16"),
+    (dict(precision=4, dims=3, sizes=[16, 16, 16], weights=np.full(27, 1 / 27.0)), "FP16"),
+    (dict(algorithm=1, precision=1), "FP64"),
+    (dict(algorithm=1, radius=2, fuse=5, weights=np.full(25, 1 / 25.0)), "fuse"),
+    (dict(algorithm=1, dims=3, radius=2, sizes=[16, 16, 16], weights=np.full(125, 1 / 125.0)), "radius"),
 ])
 def test_plan_create_unsupported(ffs, kw, needle):
     st, msg = _status(ffs, **{"sizes": [64, 64], **kw})
@@ -148,3 +154,21 @@ def test_null_arguments(ffs):
     assert lib.ffs_plan_get_info(None, None) == 1
     assert lib.ffs_plan_destroy(None) == 0
     assert lib.ffs_plan_create(None, None) == 1
+
+
+def test_c_struct_layout_matches_binding(ffs, tmp_path):
+    # sizes and field offsets of the C structs as the C compiler lays them out, against the ctypes mirror
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "ffstencil.h"\n'
+        'int main(void) {\n'
+        '  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(ffs_problem), offsetof(ffs_problem, algorithm),\n'
+        '         offsetof(ffs_problem, weights), sizeof(ffs_plan_info), offsetof(ffs_plan_info, algorithm),\n'
+        '         offsetof(ffs_plan_info, flops_per_block));\n'
+        '  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [ctypes.sizeof(ffs.Problem), ffs.Problem.algorithm.offset, ffs.Problem.weights.offset,
+            ctypes.sizeof(ffs.PlanInfo), ffs.PlanInfo.algorithm.offset, ffs.PlanInfo.flops_per_block.offset]
+    assert got == want

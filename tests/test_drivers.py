@@ -33,6 +33,18 @@ def test_usage_errors(prog):
     assert r.returncode == 1
 
 
+@pytest.mark.parametrize("flag,value", [("--algorithm", "fourier"), ("--precision", "fp8"), ("--convention", "left")])
+def test_bad_option_values_fail_before_the_gpu(flag, value):
+    # rejected by ffs_plan_create's validation, which runs before any HIP call
+    r = run("2d", "Box2D9P", "64", "4", flag, value)
+    assert r.returncode == 1 and "error" in r.stderr
+
+
+def test_help_lists_algorithms_and_forms():
+    r = run("2d", "--help")
+    assert "--algorithm" in r.stdout and "direct" in r.stdout and "f16" in r.stdout
+
+
 @pytest.mark.parametrize("prog,shape", [("1d", "Box2D9P"), ("2d", "Heat-1D"), ("3d", "NoSuchShape"), ("2d", "Box3D27P")])
 def test_wrong_shape_for_program(prog, shape):
     r = run(prog, shape, "64", "4")
@@ -65,6 +77,9 @@ CSV = re.compile(r"^FlashFFTStencil, (\S+), (\d+), (\d+), (\d+), (\d+), ([0-9.e+
     (("2d", "Box2D9P", "1000", "16"), "box_2d1r", (1, 1000, 1000)),
     (("2d", "Box2D25P", "512", "300", "32"), "box_2d2r", (1, 512, 300)),
     (("2d", "Star2D9P", "257", "8", "--precision", "bf16"), "star_2d2r", (1, 257, 257)),
+    (("2d", "Box2D25P", "700", "24", "--precision", "f16"), "box_2d2r", (1, 700, 700)),
+    (("2d", "Box2D25P", "700", "24", "--algorithm", "direct"), "box_2d2r", (1, 700, 700)),
+    (("3d", "Heat-3D", "40", "6", "--algorithm", "auto"), "star_3d1r", (40, 40, 40)),
 ])
 def test_csv_line_on_gpu(args, tag, dims):
     r = run(*args)
@@ -74,7 +89,7 @@ def test_csv_line_on_gpu(args, tag, dims):
     assert m, lines
     assert m.group(1) == tag
     assert tuple(int(m.group(i)) for i in (2, 3, 4)) == dims
-    T = int(args[-1]) if args[-1].isdigit() else int(args[3])
+    T = int(next(a for a in reversed(args[:5]) if a.isdigit()))
     ms, gst = float(m.group(6)), float(m.group(7))
     assert ms > 0
     pts = dims[0] * dims[1] * dims[2]
