@@ -27,3 +27,22 @@ def test_autotune_picks_admissible_and_caches(ffs, oracle, torch_cuda, tmp_path,
     plan.close()
     tol = 1e-12 if form == "fp64" else 1e-6
     assert oracle.errors(got, oracle.sweep(x, r, w, steps))[1] <= tol
+
+
+@pytest.mark.parametrize("shape,size,steps,form", [("Box2D25P", (300, 280), 12, "f16"), ("Heat-3D", (40, 36, 50), 6, "fp64")])
+def test_autotune_across_algorithms(ffs, oracle, torch_cuda, tmp_path, shape, size, steps, form):
+    # algorithm "auto": FFT depths and direct depths are all timed; the winner's plan meets the bar of
+    # its own arithmetic (FP64 for a direct winner)
+    dims, r, w, _ = ffs.named_stencil(shape, seed=6)
+    cache = str(tmp_path / "tune.jsonl")
+    algo, tile, fuse, ms = ffs.autotune_full(dims, size, w, r, steps, precision=form, cache=cache, algorithm="auto")
+    assert algo in ("fft", "direct") and fuse >= 1 and ms > 0
+    assert ffs.autotune_full(dims, size, w, r, steps, precision=form, cache=cache, algorithm="auto")[:3] == \
+        (algo, tile, fuse)
+    prec = "fp64" if algo == "direct" else form
+    x = oracle.uniform(10, size)
+    plan = ffs.Plan(dims, size, w, r, steps, precision=prec, tile=tile, fuse=fuse, algorithm=algo)
+    got = plan.run_host(x)
+    plan.close()
+    tol = 1e-12 if prec == "fp64" else 1e-6
+    assert oracle.errors(got, oracle.sweep(x, r, w, steps))[1] <= tol
