@@ -791,8 +791,17 @@ __device__ __forceinline__ h8 as_h8(u4 w) { return __builtin_bit_cast(h8, w); }
 __device__ __forceinline__ uint32_t pk_f16(float a, float b) {
   return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{a, b}, h2));
 }
-__device__ __forceinline__ float f16_lo(uint32_t w) { return (float)__builtin_bit_cast(h2, w)[0]; }
-__device__ __forceinline__ float f16_hi(uint32_t w) { return (float)__builtin_bit_cast(h2, w)[1]; }
+
+// The FP16 residuals (a - h.lo, b - h.hi) of one word, each rounded once to FP16: v_fma_mixlo_f16 and
+// v_fma_mixhi_f16 widen the FP16 half, subtract it from the FP32 value and round the difference, in one
+// instruction per value (bit-identical to converting back, subtracting in FP32 and converting, which
+// takes five instructions per word; checked over 2^20 random pairs spanning 40 binades).
+__device__ __forceinline__ uint32_t resid_f16(uint32_t hw, float a, float b) {
+  uint32_t lw;
+  asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(lw) : "v"(hw), "v"(a));
+  asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(lw) : "v"(hw), "v"(b));
+  return lw;
+}
 
 // Same element order as split8: word k of the operand holds elements (2k, 2k+1) of (x0 | x1).
 __device__ __forceinline__ void split8_f16(const f4& x0, const f4& x1, h8& h, h8& l) {
@@ -802,7 +811,7 @@ __device__ __forceinline__ void split8_f16(const f4& x0, const f4& x1, h8& h, h8
     const f4& x = k < 2 ? x0 : x1;
     const int e = 2 * (k & 1);
     hw[k] = pk_f16(x[e], x[e + 1]);
-    lw[k] = pk_f16(x[e] - f16_lo(hw[k]), x[e + 1] - f16_hi(hw[k]));
+    lw[k] = resid_f16(hw[k], x[e], x[e + 1]);
   }
   h = as_h8(hw);
   l = as_h8(lw);
