@@ -37,6 +37,7 @@ STATUS_NAMES = {
 PRECISIONS = {"fp64": 0, "bf16": 1, "mixed": 1, "f32": 2, "bf16x2": 3, "f16": 4}
 CONVENTIONS = {"ref": 0, "centred": 1, "centered": 1}
 ALGORITHMS = {"fft": 0, "direct": 1, "auto": 2}
+MAPPINGS = {"auto": 0, "xcd": 1, "linear": 2}
 
 
 class FFSError(RuntimeError):
@@ -59,7 +60,8 @@ class Problem(ctypes.Structure):
         ("use_graph", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("algorithm", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("mapping", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -81,7 +83,8 @@ class PlanInfo(ctypes.Structure):
         ("flops_per_block", ctypes.c_double),
         ("bytes_per_block", ctypes.c_double),
         ("algorithm", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("mapping", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
     def as_dict(self) -> dict:
@@ -198,7 +201,7 @@ class Plan:
 
     def __init__(self, dims: int, sizes: Sequence[int], weights, radius: int, steps: int,
                  precision: str = "fp64", convention: str = "ref", tile: int = 0, fuse: int = 0,
-                 use_graph: bool = True, device: int = 0, algorithm: str = "fft"):
+                 use_graph: bool = True, device: int = 0, algorithm: str = "fft", mapping: str = "auto"):
         lib = load_library()
         self._w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).ravel())
         p = Problem()
@@ -217,6 +220,7 @@ class Plan:
         p.use_graph = 1 if use_graph else 0
         p.device = device
         p.algorithm = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+        p.mapping = MAPPINGS[mapping] if isinstance(mapping, str) else int(mapping)
         self.problem = p
         self.sizes = tuple(sz[:dims])
         self.dims = dims
@@ -270,14 +274,14 @@ def autotune(dims: int, sizes: Sequence[int], weights, radius: int, steps: int, 
              convention: str = "ref", cache: Optional[str] = None, algorithm: str = "fft") -> Tuple[int, int, float]:
     """Measured best (tile, fuse, ms per run) for this problem on device 0 (cached in `cache`).  With
     algorithm "auto" the winner's algorithm is in :func:`autotune_full`."""
-    a, tile, fuse, ms = autotune_full(dims, sizes, weights, radius, steps, precision, convention, cache, algorithm)
-    return tile, fuse, ms
+    r = autotune_full(dims, sizes, weights, radius, steps, precision, convention, cache, algorithm)
+    return r[1], r[2], r[4]
 
 
 def autotune_full(dims: int, sizes: Sequence[int], weights, radius: int, steps: int, precision: str = "fp64",
                   convention: str = "ref", cache: Optional[str] = None,
-                  algorithm: str = "fft") -> Tuple[str, int, int, float]:
-    """Measured best (algorithm, tile, fuse, ms per run)."""
+                  algorithm: str = "fft") -> Tuple[str, int, int, str, float]:
+    """Measured best (algorithm, tile, fuse, mapping, ms per run)."""
     lib = load_library()
     w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).ravel())
     p = Problem()
@@ -292,7 +296,8 @@ def autotune_full(dims: int, sizes: Sequence[int], weights, radius: int, steps: 
     ms = ctypes.c_double()
     _check(lib.ffs_autotune(ctypes.byref(p), cache.encode() if cache else None, ctypes.byref(ms)))
     name = {v: k for k, v in ALGORITHMS.items()}[p.algorithm]
-    return name, p.tile, p.fuse, ms.value
+    mname = {0: "auto", 1: "xcd", 2: "linear"}[p.mapping]
+    return name, p.tile, p.fuse, mname, ms.value
 
 
 class Batch:

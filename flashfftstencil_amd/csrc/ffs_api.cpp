@@ -125,6 +125,8 @@ ffs_status validate(const ffs_problem* p, int64_t* npoints) {
   if (p->fuse < 0) return fail(FFS_ERR_INVALID_ARG, "fuse must be >= 0 (got %d)", p->fuse);
   if (p->algorithm < FFS_ALGO_FFT || p->algorithm > FFS_ALGO_AUTO)
     return fail(FFS_ERR_INVALID_ARG, "unknown algorithm %d", p->algorithm);
+  if (p->mapping < FFS_MAP_AUTO || p->mapping > FFS_MAP_LINEAR)
+    return fail(FFS_ERR_INVALID_ARG, "unknown mapping %d", p->mapping);
   int64_t n = 1;
   for (int d = 0; d < p->dims; d++) {
     if (p->sizes[d] < 1) return fail(FFS_ERR_INVALID_ARG, "size[%d] must be >= 1 (got %lld)", d, (long long)p->sizes[d]);
@@ -354,6 +356,7 @@ ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, boo
   a.h2 = h[2];
   a.precision = p.precision;
   a.scale_exp = tail ? pl->scale_tail : pl->scale_main;
+  a.linear_map = p.mapping == FFS_MAP_LINEAR ? 1 : 0;
   return a;
 }
 
@@ -658,6 +661,7 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   info->precision = p.precision;
   info->convention = p.convention;
   info->algorithm = pl->direct ? FFS_ALGO_DIRECT : FFS_ALGO_FFT;
+  info->mapping = p.mapping == FFS_MAP_LINEAR ? FFS_MAP_LINEAR : FFS_MAP_XCD;
   if (pl->direct) {
     const ffsk::DirectArgs a = make_direct_args(pl, nullptr, nullptr, pl->t);
     const int r = p.radius, K = 2 * r + 1;
@@ -836,40 +840,43 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
   std::snprintf(key, sizeof key, "d%d_r%d_p%d_c%d_a%d_n%lldx%lldx%lld_T%lld", p->dims, p->radius, p->precision,
                 p->convention, p->algorithm, (long long)p->sizes[0], (long long)(p->dims > 1 ? p->sizes[1] : 1),
                 (long long)(p->dims > 2 ? p->sizes[2] : 1), (long long)p->steps);
-  auto apply = [&](int algo, int tile, int fuse) {
+  auto apply = [&](int algo, int tile, int fuse, int mapping) {
     p->algorithm = algo;
     if (algo == FFS_ALGO_DIRECT) p->precision = FFS_PREC_FP64;
     p->tile = algo == FFS_ALGO_DIRECT ? 0 : tile;
     p->fuse = fuse;
+    p->mapping = mapping;
   };
-  if (cache_path) {  // a line: {"key": "...", "algorithm": A, "tile": P, "fuse": t, "ms": x}
+  if (cache_path) {  // a line: {"key": "...", "algorithm": A, "tile": P, "fuse": t, "mapping": M, "ms": x}
     if (FILE* f = std::fopen(cache_path, "r")) {
       char line[512];
-      int algo = -1, tile = -1, fuse = -1;
+      int algo = -1, tile = -1, fuse = -1, mapping = 0;
       double ms = 0;
       while (std::fgets(line, sizeof line, f)) {
         char k[256];
-        int al, tl, fu;
+        int al, tl, fu, mp;
         double m;
-        if (std::sscanf(line, " {\"key\": \"%255[^\"]\", \"algorithm\": %d, \"tile\": %d, \"fuse\": %d, \"ms\": %lf", k,
-                        &al, &tl, &fu, &m) == 5 &&
+        if (std::sscanf(line,
+                        " {\"key\": \"%255[^\"]\", \"algorithm\": %d, \"tile\": %d, \"fuse\": %d, \"mapping\": %d, \"ms\": %lf",
+                        k, &al, &tl, &fu, &mp, &m) == 6 &&
             std::strcmp(k, key) == 0) {
           algo = al;
           tile = tl;
           fuse = fu;
+          mapping = mp;
           ms = m;
         }
       }
       std::fclose(f);
       if (algo >= 0) {
-        apply(algo, tile, fuse);
+        apply(algo, tile, fuse, mapping);
         if (best_ms) *best_ms = ms;
         return FFS_OK;
       }
     }
   }
   struct Cand {
-    int algo, tile, fuse;
+    int algo, tile, fuse, mapping;
   };
   std::vector<Cand> cands;
   const int64_t tcap = std::max<int64_t>(p->steps, 1);
@@ -885,7 +892,8 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
       const int vmin = p->dims == 1 ? 16 : 2;
       const int64_t tmax = p->radius == 0 ? 1 : std::max<int64_t>(1, (tile - vmin) / (2 * p->radius));
       for (int64_t t = 1; t <= std::min(tmax, tcap); t++)
-        if (pick(t, tmax)) cands.push_back({FFS_ALGO_FFT, tile, (int)t});
+        if (pick(t, tmax))
+          for (int mp : {FFS_MAP_XCD, FFS_MAP_LINEAR}) cands.push_back({FFS_ALGO_FFT, tile, (int)t, mp});
     }
   }
   if (p->algorithm != FFS_ALGO_FFT) {  // direct candidates (FP64), when the radius is offered
@@ -896,10 +904,10 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
     probe.weights.assign(p->weights, p->weights + ntaps);
     const int64_t tmax = ffsk::direct_max_depth(p->dims, p->radius, star_support(&probe));
     for (int64_t t = 1; t <= std::min(tmax, tcap); t++)
-      if (pick(t, tmax)) cands.push_back({FFS_ALGO_DIRECT, 0, (int)t});
+      if (pick(t, tmax)) cands.push_back({FFS_ALGO_DIRECT, 0, (int)t, FFS_MAP_AUTO});
   }
   double best = 1e300;
-  Cand win{-1, 0, 0};
+  Cand win{-1, 0, 0, 0};
   double *din = nullptr, *dout = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   auto cleanup = [&]() {
@@ -914,6 +922,7 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
     if (cd.algo == FFS_ALGO_DIRECT) c.precision = FFS_PREC_FP64;
     c.tile = cd.tile;
     c.fuse = cd.fuse;
+    c.mapping = cd.mapping;
     ffs_plan* pl = nullptr;
     if (ffs_plan_create(&c, &pl) != FFS_OK) continue;
     if (!din) {
@@ -951,12 +960,12 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
   }
   cleanup();
   if (win.algo < 0) return fail(FFS_ERR_UNSUPPORTED, "no admissible configuration");
-  apply(win.algo, win.tile, win.fuse);
+  apply(win.algo, win.tile, win.fuse, win.mapping);
   if (best_ms) *best_ms = best;
   if (cache_path) {
     if (FILE* f = std::fopen(cache_path, "a")) {
-      std::fprintf(f, "{\"key\": \"%s\", \"algorithm\": %d, \"tile\": %d, \"fuse\": %d, \"ms\": %.6f}\n", key,
-                   win.algo, win.tile, win.fuse, best);
+      std::fprintf(f, "{\"key\": \"%s\", \"algorithm\": %d, \"tile\": %d, \"fuse\": %d, \"mapping\": %d, \"ms\": %.6f}\n",
+                   key, win.algo, win.tile, win.fuse, win.mapping, best);
       std::fclose(f);
     }
   }

@@ -55,6 +55,8 @@ struct TileArgs {
   int32_t precision;
   int32_t scale_exp;  // FP16 form: the tile pair is scaled so that its largest fluctuation is below 2^scale_exp
   double dc_gain;  // K(0)^t: gain of the constant mode (mixed forms add mean * gain back in FP64)
+  int32_t linear_map;  // 1: workgroups take tiles in hardware order; 0: XCD-contiguous bands
+  int32_t pad_;
 };
 
 // Direct (non-FFT) FP64 kernels with temporal blocking (ffs_direct.hip).

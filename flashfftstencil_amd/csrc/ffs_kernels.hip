@@ -67,6 +67,10 @@ __device__ __forceinline__ int64_t xcd_logical_block(int64_t b, int64_t G) {
   const int64_t start = x * per + (x < rem ? x : rem);
   return start + (b >> 3);
 }
+// The plan's wave-to-tile mapping: XCD-contiguous bands (default) or the hardware order.
+__device__ __forceinline__ int64_t block_order(int32_t linear, int64_t b, int64_t G) {
+  return linear ? b : xcd_logical_block(b, G);
+}
 
 struct OpsAxis {
   double r[4], s[4], d[4];
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
-  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
   // 2D: both axes are 16-point DFTs and share one operand set; 1D: 15-point rows, 16-point columns.
   OpsAxis f0, f1;
@@ -554,7 +558,7 @@ __global__ __launch_bounds__(256) void k_tile16_mixed(TileArgs a) {
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
-  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
   Ops f0, f1;
   load_ops_mixed(f0, a.opsm, 0, lane);
@@ -926,7 +930,7 @@ __global__ __launch_bounds__(256, 3) void k_tile32_bf16(TileArgs a) {
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
-  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair<NTERMS, true>(a, tp, sF, sH, lane, q, j);
 }
 
@@ -946,7 +950,7 @@ __global__ __launch_bounds__(256, 3) void k_batch32_bf16(const TileArgs* __restr
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
-  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t gw = block_order(items[0].linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t total = prefix[n];
   for (int64_t idx = gw; idx < total; idx += W) {
     int lo = 0, hi = n;  // largest g with prefix[g] <= idx
@@ -969,7 +973,7 @@ __global__ __launch_bounds__(256, 3) void k_tile32_f16(TileArgs a) {
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
-  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair_f16<true>(a, tp, sF, sH, lane, q, j);
 }
 
@@ -981,7 +985,7 @@ __global__ __launch_bounds__(256, 3) void k_batch32_f16(const TileArgs* __restri
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
-  const int64_t gw = xcd_logical_block(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t gw = block_order(items[0].linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t total = prefix[n];
   for (int64_t idx = gw; idx < total; idx += W) {
     int lo = 0, hi = n;
@@ -1090,7 +1094,7 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
   const bool mixed = PREC != 0;
   const int64_t pairs2 = (a.tiles2 + 1) >> 1;
 
-  for (int64_t tp = xcd_logical_block(blockIdx.x, gridDim.x); tp < a.pairs; tp += gridDim.x) {
+  for (int64_t tp = block_order(a.linear_map, blockIdx.x, gridDim.x); tp < a.pairs; tp += gridDim.x) {
     const int64_t T0 = tp / (a.tiles1 * pairs2);
     const int64_t rem = tp - T0 * (a.tiles1 * pairs2);
     const int64_t T1 = rem / pairs2, T2 = (rem - T1 * pairs2) * 2;

@@ -53,6 +53,7 @@ static void print_help() {
       "  --convention C       ref (default, out[p] = sum_a w[a] in[p-a]) | centred (out[p] = sum_o w[o+r] in[p+o])\n"
       "  --algorithm A        fft (default: tiled FFT stencil) | direct (FP64 stencil sum, temporal blocking) |\n"
       "                       auto (direct where offered)\n"
+      "  --mapping M          xcd (default: each XCD takes a contiguous band of tiles) | linear\n"
       "  --seed S             seed of the weights and the initial grid (default 0x5EED0000)\n"
       "  --fuse t             fused time depth (default automatic)\n"
       "  --no-graph           plain launches instead of a hipGraph replay\n"
@@ -79,7 +80,7 @@ static bool parse_i64(const char* s, long long* v) {
 
 int main(int argc, char** argv) {
   std::vector<std::string> pos;
-  std::string custom, prec = "fp64", conv = "ref", algo = "fft";
+  std::string custom, prec = "fp64", conv = "ref", algo = "fft", mapping = "xcd";
   unsigned long long seed = 0x5EED0000ULL;
   int fuse = 0, use_graph = 1;
   bool tune = false;
@@ -105,6 +106,8 @@ int main(int argc, char** argv) {
       conv = need("--convention");
     } else if (a == "--algorithm") {
       algo = need("--algorithm");
+    } else if (a == "--mapping") {
+      mapping = need("--mapping");
     } else if (a == "--seed") {
       seed = std::strtoull(need("--seed"), nullptr, 0);
     } else if (a == "--fuse") {
@@ -193,6 +196,7 @@ int main(int argc, char** argv) {
                                  : -1;
   p.convention = conv == "ref" ? FFS_CONV_REF : conv == "centred" || conv == "centered" ? FFS_CONV_CENTRED : -1;
   p.algorithm = algo == "fft" ? FFS_ALGO_FFT : algo == "direct" ? FFS_ALGO_DIRECT : algo == "auto" ? FFS_ALGO_AUTO : -1;
+  p.mapping = mapping == "xcd" ? FFS_MAP_XCD : mapping == "linear" ? FFS_MAP_LINEAR : -1;
   p.fuse = fuse;
   p.use_graph = use_graph;
 

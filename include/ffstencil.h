@@ -80,6 +80,14 @@ typedef enum ffs_convention { FFS_CONV_REF = 0, FFS_CONV_CENTRED = 1 } ffs_conve
  *           A DIRECT result is FP64 whatever precision was asked. */
 typedef enum ffs_algorithm { FFS_ALGO_FFT = 0, FFS_ALGO_DIRECT = 1, FFS_ALGO_AUTO = 2 } ffs_algorithm;
 
+/* Wave-to-tile mapping of the FFT tile kernels: which workgroup takes which tile pairs.
+ *   AUTO / XCD: workgroups are renumbered so that each of the 8 XCDs takes one contiguous band of tile
+ *               pairs (neighbouring windows share that XCD's L2);
+ *   LINEAR:     hardware dispatch order (consecutive workgroups go to different XCDs).
+ * The direct kernels ignore it.  No reference counterpart (the reference launches one warp per tile
+ * pair in grid order). */
+typedef enum ffs_mapping { FFS_MAP_AUTO = 0, FFS_MAP_XCD = 1, FFS_MAP_LINEAR = 2 } ffs_mapping;
+
 typedef struct ffs_problem {
   int32_t dims;          /* 1, 2 or 3 */
   int32_t radius;        /* r >= 0; taps per axis K = 2r+1 */
@@ -94,7 +102,8 @@ typedef struct ffs_problem {
                             shorter chains launch directly, which is cheaper), 0 = plain launches */
   int32_t device;        /* HIP device ordinal */
   int32_t algorithm;     /* ffs_algorithm (default FFS_ALGO_FFT) */
-  int32_t reserved[3];
+  int32_t mapping;       /* ffs_mapping (default FFS_MAP_AUTO) */
+  int32_t reserved[2];
 } ffs_problem;
 
 /* What a plan chose. */
@@ -114,7 +123,8 @@ typedef struct ffs_plan_info {
                              comment) or FP64 FMA flops including the halo recomputation (DIRECT) */
   double bytes_per_block; /* algorithmic HBM bytes of one block: 16 B per grid point */
   int32_t algorithm;      /* ffs_algorithm actually used (FFT or DIRECT) */
-  int32_t reserved[3];
+  int32_t mapping;        /* ffs_mapping used by the tile kernels (XCD or LINEAR) */
+  int32_t reserved[2];
 } ffs_plan_info;
 
 typedef struct ffs_plan ffs_plan;
@@ -150,10 +160,10 @@ ffs_status ffs_run_host(ffs_plan* plan, const double* h_in, double* h_out);
  * d_out = S^t(d_in).  One kernel launch; used to time the tile kernel alone. */
 ffs_status ffs_run_block(ffs_plan* plan, const double* d_in, double* d_out, int32_t tail, void* stream);
 
-/* Autotuner: time the admissible (algorithm, tile, fused depth) plans of `p` on its device (full
- * T-step runs on a scratch grid of p's sizes) and write the fastest into p->tile, p->fuse and
- * p->algorithm (FFS_ALGO_AUTO searches both algorithms; a DIRECT winner also sets p->precision to
- * FFS_PREC_FP64).  With a non-NULL `cache_path`, results are read from / appended to that JSON-lines
+/* Autotuner: time the admissible (algorithm, tile, fused depth, wave-to-tile mapping) plans of `p` on
+ * its device (full T-step runs on a scratch grid of p's sizes) and write the fastest into p->tile,
+ * p->fuse, p->mapping and p->algorithm (FFS_ALGO_AUTO searches both algorithms; a DIRECT winner also
+ * sets p->precision to FFS_PREC_FP64).  With a non-NULL `cache_path`, results are read from / appended to that JSON-lines
  * file keyed by (dims, radius, precision, convention, algorithm, sizes, T), so each configuration is
  * measured once.  best_ms (may be NULL) receives the winner's time per run.  No reference counterpart
  * (the reference hard-codes its tile). */

@@ -40,8 +40,8 @@ def test_library_exports_every_declared_symbol(ffs):
 
 
 def test_struct_layouts_match_header(ffs):
-    # ffs_problem: 2*i32, 3*i64, ptr, i64, 7*i32 (algorithm last), 3*i32 reserved
-    assert ctypes.sizeof(ffs.Problem) == 4 * 2 + 8 * 3 + 8 + 8 + 4 * 7 + 4 * 3
+    # ffs_problem: 2*i32, 3*i64, ptr, i64, 8*i32 (algorithm, mapping last), 2*i32 reserved
+    assert ctypes.sizeof(ffs.Problem) == 4 * 2 + 8 * 3 + 8 + 8 + 4 * 8 + 4 * 2
     lib = ffs.load_library()
     p = ffs.Problem()
     assert lib.ffs_problem_init(ctypes.byref(p)) == 0
@@ -123,7 +123,7 @@ def _status(ffs, **kw):
     (dict(dims=0), "dims"), (dict(dims=4), "dims"), (dict(radius=-1), "radius"), (dict(steps=-3), "steps"),
     (dict(sizes=[0, 5]), "size"), (dict(weights=None), "weights"), (dict(precision=9), "precision"),
     (dict(convention=5), "convention"), (dict(fuse=-1), "fuse"), (dict(algorithm=3), "algorithm"),
-    (dict(algorithm=-1), "algorithm"),
+    (dict(algorithm=-1), "algorithm"), (dict(mapping=3), "mapping"),
     (dict(weights=np.array([1.0, np.nan] + [0.0] * 7)), "finite"),
 ])
 def test_plan_create_rejects_invalid(ffs, kw, needle):
@@ -162,13 +162,15 @@ def test_c_struct_layout_matches_binding(ffs, tmp_path):
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "ffstencil.h"\n'
         'int main(void) {\n'
-        '  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(ffs_problem), offsetof(ffs_problem, algorithm),\n'
+        '  printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ffs_problem), offsetof(ffs_problem, algorithm),\n'
         '         offsetof(ffs_problem, weights), sizeof(ffs_plan_info), offsetof(ffs_plan_info, algorithm),\n'
-        '         offsetof(ffs_plan_info, flops_per_block));\n'
+        '         offsetof(ffs_plan_info, flops_per_block), offsetof(ffs_problem, mapping),\n'
+        '         offsetof(ffs_plan_info, mapping));\n'
         '  return 0;\n}\n')
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(ffs.Problem), ffs.Problem.algorithm.offset, ffs.Problem.weights.offset,
-            ctypes.sizeof(ffs.PlanInfo), ffs.PlanInfo.algorithm.offset, ffs.PlanInfo.flops_per_block.offset]
+            ctypes.sizeof(ffs.PlanInfo), ffs.PlanInfo.algorithm.offset, ffs.PlanInfo.flops_per_block.offset,
+            ffs.Problem.mapping.offset, ffs.PlanInfo.mapping.offset]
     assert got == want

@@ -33,7 +33,8 @@ def test_usage_errors(prog):
     assert r.returncode == 1
 
 
-@pytest.mark.parametrize("flag,value", [("--algorithm", "fourier"), ("--precision", "fp8"), ("--convention", "left")])
+@pytest.mark.parametrize("flag,value", [("--algorithm", "fourier"), ("--precision", "fp8"), ("--convention", "left"),
+                                        ("--mapping", "spiral")])
 def test_bad_option_values_fail_before_the_gpu(flag, value):
     # rejected by ffs_plan_create's validation, which runs before any HIP call
     r = run("2d", "Box2D9P", "64", "4", flag, value)

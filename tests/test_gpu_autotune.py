@@ -19,6 +19,7 @@ def test_autotune_picks_admissible_and_caches(ffs, oracle, torch_cuda, tmp_path,
     assert tile in ((240,) if dims == 1 else ((16, 32) if form == "bf16" else (16,)))
     lines = [json.loads(l) for l in open(cache)]
     assert len(lines) == 1 and lines[0]["tile"] == tile and lines[0]["fuse"] == fuse
+    assert lines[0]["mapping"] in (1, 2)
     assert ffs.autotune(dims, size, w, r, steps, precision=form, cache=cache)[:2] == (tile, fuse)
     assert len(open(cache).readlines()) == 1  # served from the cache
     x = oracle.uniform(9, size)
@@ -35,13 +36,14 @@ def test_autotune_across_algorithms(ffs, oracle, torch_cuda, tmp_path, shape, si
     # its own arithmetic (FP64 for a direct winner)
     dims, r, w, _ = ffs.named_stencil(shape, seed=6)
     cache = str(tmp_path / "tune.jsonl")
-    algo, tile, fuse, ms = ffs.autotune_full(dims, size, w, r, steps, precision=form, cache=cache, algorithm="auto")
-    assert algo in ("fft", "direct") and fuse >= 1 and ms > 0
-    assert ffs.autotune_full(dims, size, w, r, steps, precision=form, cache=cache, algorithm="auto")[:3] == \
-        (algo, tile, fuse)
+    algo, tile, fuse, mapping, ms = ffs.autotune_full(dims, size, w, r, steps, precision=form, cache=cache,
+                                                      algorithm="auto")
+    assert algo in ("fft", "direct") and fuse >= 1 and ms > 0 and mapping in ("auto", "xcd", "linear")
+    assert ffs.autotune_full(dims, size, w, r, steps, precision=form, cache=cache, algorithm="auto")[:4] == \
+        (algo, tile, fuse, mapping)
     prec = "fp64" if algo == "direct" else form
     x = oracle.uniform(10, size)
-    plan = ffs.Plan(dims, size, w, r, steps, precision=prec, tile=tile, fuse=fuse, algorithm=algo)
+    plan = ffs.Plan(dims, size, w, r, steps, precision=prec, tile=tile, fuse=fuse, algorithm=algo, mapping=mapping)
     got = plan.run_host(x)
     plan.close()
     tol = 1e-12 if prec == "fp64" else 1e-6

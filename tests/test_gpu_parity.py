@@ -147,3 +147,22 @@ def test_3d_fp64_fuse_convention(ffs, oracle, torch_cuda, convention, fuse):
     got, info = _run(ffs, torch_cuda, x, 1, w, 2 * fuse + 1, convention=convention, fuse=fuse)
     ref = oracle.sweep(x, 1, w, 2 * fuse + 1, convention=0 if convention == "ref" else 1)
     assert oracle.errors(got, ref)[1] <= TOL_FP64, info
+
+
+@pytest.mark.parametrize("precision", ["fp64", "f16", "bf16"])
+def test_linear_mapping_is_bitwise_equal(ffs, oracle, torch_cuda, precision):
+    # the wave-to-tile mapping only reorders independent tile pairs: results are bitwise identical
+    torch = torch_cuda
+    _, r, w, _ = ffs.named_stencil("Box2D25P", seed=8)
+    x = oracle.uniform(31, (333, 290))
+    d_in = torch.from_numpy(x).cuda()
+    outs = []
+    for mapping in ("xcd", "linear"):
+        plan = ffs.Plan(2, x.shape, w, r, 11, precision=precision, mapping=mapping)
+        assert plan.info["mapping"] == (1 if mapping == "xcd" else 2)
+        d_out = torch.empty_like(d_in)
+        plan.run(d_in, d_out)
+        torch.cuda.synchronize()
+        outs.append(d_out.cpu().numpy())
+        plan.close()
+    assert np.array_equal(outs[0], outs[1])
