@@ -1067,6 +1067,8 @@ ffs_status ffs_batch_run(ffs_batch* b, const double* const* d_in, double* const*
       prefix[(size_t)(r * (n + 1) + k)] = acc;
       b->active[(size_t)r] = k;
     }
+    // a replay of the previous graph may still be reading the work lists (on any stream): let it finish
+    FFS_HIP(hipDeviceSynchronize());
     FFS_HIP(hipMemcpy(b->d_items, items.data(), sizeof(ffsk::TileArgs) * items.size(), hipMemcpyHostToDevice));
     FFS_HIP(hipMemcpy(b->d_prefix, prefix.data(), sizeof(int64_t) * prefix.size(), hipMemcpyHostToDevice));
     if (b->gexec) {
