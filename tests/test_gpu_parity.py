@@ -166,3 +166,35 @@ def test_linear_mapping_is_bitwise_equal(ffs, oracle, torch_cuda, precision):
         outs.append(d_out.cpu().numpy())
         plan.close()
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("algorithm,precision", [("fft", "fp64"), ("fft", "f16"), ("direct", "fp64")])
+def test_run_captured_in_caller_graph(ffs, oracle, torch_cuda, algorithm, precision):
+    # with use_graph = 0 a run only enqueues kernels on the caller's stream, so it can be recorded into the
+    # caller's own graph; every replay (with eager work in between) must equal the eager result
+    torch = torch_cuda
+    _, r, w, _ = ffs.named_stencil("Box2D25P", seed=12)
+    x = oracle.uniform(41, (200, 180))
+    d_in = torch.from_numpy(x).cuda()
+    plan = ffs.Plan(2, x.shape, w, r, 9, precision=precision, algorithm=algorithm, use_graph=False)
+    eager = torch.empty_like(d_in)
+    plan.run(d_in, eager)
+    torch.cuda.synchronize()
+    out = torch.empty_like(d_in)
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        plan.run(d_in, out)  # warm-up on the side stream before capture
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        plan.run(d_in, out)
+    for k in range(3):
+        out.zero_()
+        scratch = torch.ones(1000, device="cuda") * k  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), (k, float(scratch.sum()))
+    plan.close()
