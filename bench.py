@@ -124,7 +124,7 @@ def log(*a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed full runs")
+    ap.add_argument("--steps", type=int, default=5, help="timed full runs")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="box2d25p", choices=sorted(WORKLOADS))
     ap.add_argument("--kernel-reps", type=int, default=10, help="timed single-block launches for the roofline")
