@@ -122,6 +122,7 @@ def load_library() -> ctypes.CDLL:
         "ffs_stencil_named": [ctypes.c_char_p, ctypes.c_uint64, P(ctypes.c_int32), P(ctypes.c_int32),
                               P(ctypes.c_double), ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32],
         "ffs_fill_uniform": [ctypes.c_uint64, P(ctypes.c_double), ctypes.c_int64],
+        "ffs_fill_uniform_device": [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p],
         "ffs_plan_create": [P(Problem), P(ctypes.c_void_p)],
         "ffs_plan_get_info": [ctypes.c_void_p, P(PlanInfo)],
         "ffs_plan_destroy": [ctypes.c_void_p],
@@ -177,6 +178,13 @@ def fill_uniform(seed: int, n: int) -> np.ndarray:
     out = np.empty(int(n), dtype=np.float64)
     _check(load_library().ffs_fill_uniform(ctypes.c_uint64(seed), _dptr(out), ctypes.c_int64(out.size)))
     return out
+
+
+def fill_uniform_device(seed: int, d_out, stream=None) -> None:
+    """Write the ffs_fill_uniform sequence into a float64 CUDA tensor (or device pointer with .numel)."""
+    n = d_out.numel()
+    _check(load_library().ffs_fill_uniform_device(ctypes.c_uint64(seed), ctypes.c_void_p(_ptr(d_out)),
+                                                   ctypes.c_int64(n), ctypes.c_void_p(_stream_handle(stream))))
 
 
 def format_csv(tag: str, sizes: Sequence[int], steps: int, ms: float) -> str:

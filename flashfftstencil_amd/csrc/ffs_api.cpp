@@ -356,7 +356,7 @@ ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, boo
   a.h2 = h[2];
   a.precision = p.precision;
   a.scale_exp = tail ? pl->scale_tail : pl->scale_main;
-  a.linear_map = p.mapping == FFS_MAP_LINEAR ? 1 : 0;
+  a.linear_map = p.mapping == FFS_MAP_XCD ? 0 : 1;  // AUTO = hardware order (measured 1-3% faster)
   return a;
 }
 
@@ -500,6 +500,14 @@ ffs_status ffs_fill_uniform(uint64_t seed, double* host_out, int64_t n) {
   if (n < 0) return fail(FFS_ERR_INVALID_ARG, "negative count");
   uint64_t s = seed;
   for (int64_t i = 0; i < n; i++) host_out[i] = uniform01(s);
+  return FFS_OK;
+}
+
+ffs_status ffs_fill_uniform_device(uint64_t seed, double* d_out, int64_t n, void* stream) {
+  if (!d_out || n < 0) return fail(FFS_ERR_INVALID_ARG, "bad device buffer or size");
+  if (n == 0) return FFS_OK;
+  const hipError_t e = ffsk::fill_uniform_device(d_out, n, seed, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(FFS_ERR_HIP, "fill kernel launch failed: %s", hipGetErrorString(e));
   return FFS_OK;
 }
 
@@ -661,7 +669,7 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   info->precision = p.precision;
   info->convention = p.convention;
   info->algorithm = pl->direct ? FFS_ALGO_DIRECT : FFS_ALGO_FFT;
-  info->mapping = p.mapping == FFS_MAP_LINEAR ? FFS_MAP_LINEAR : FFS_MAP_XCD;
+  info->mapping = p.mapping == FFS_MAP_XCD ? FFS_MAP_XCD : FFS_MAP_LINEAR;
   if (pl->direct) {
     const ffsk::DirectArgs a = make_direct_args(pl, nullptr, nullptr, pl->t);
     const int r = p.radius, K = 2 * r + 1;
@@ -928,7 +936,8 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
     if (!din) {
       hipError_t e = hipMalloc(&din, sizeof(double) * npoints);
       if (e == hipSuccess) e = hipMalloc(&dout, sizeof(double) * npoints);
-      if (e == hipSuccess) e = hipMemset(din, 0, sizeof(double) * npoints);
+      // time on data like the caller's: an all-zero grid runs at a higher clock (less switching)
+      if (e == hipSuccess) e = ffsk::fill_uniform_device(din, npoints, 0x5EED0000ULL, nullptr);
       if (e == hipSuccess) e = hipEventCreate(&e0);
       if (e == hipSuccess) e = hipEventCreate(&e1);
       if (e != hipSuccess) {

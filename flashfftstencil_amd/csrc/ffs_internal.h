@@ -88,6 +88,8 @@ hipError_t launch_direct(const DirectArgs& a, int dims, int radius, bool star, h
 // Launch helpers implemented in ffs_kernels.hip.
 hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hipStream_t s);
 hipError_t gen_operands32(uint32_t* ops32, bool f16, hipStream_t s);
+// d[i] = output i of SplitMix64(seed) as a double in [0, 1) (the host ffs_fill_uniform sequence)
+hipError_t fill_uniform_device(double* d, int64_t n, uint64_t seed, hipStream_t s);
 // layout 0: FP64 accumulator order (row freq q + 4v); layout 1: FP32 accumulator order (row freq 4q + v)
 hipError_t gen_spectrum(double* H, int kind, int layout, const int32_t* axis_len, const double* d_taps,
                         int32_t ntaps_per_axis, int32_t dims, int32_t convention, int32_t radius, int64_t t,

@@ -67,7 +67,7 @@ __device__ __forceinline__ int64_t xcd_logical_block(int64_t b, int64_t G) {
   const int64_t start = x * per + (x < rem ? x : rem);
   return start + (b >> 3);
 }
-// The plan's wave-to-tile mapping: XCD-contiguous bands (default) or the hardware order.
+// The plan's wave-to-tile mapping: the hardware order (default) or XCD-contiguous bands.
 __device__ __forceinline__ int64_t block_order(int32_t linear, int64_t b, int64_t G) {
   return linear ? b : xcd_logical_block(b, G);
 }
@@ -1434,6 +1434,23 @@ hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hi
   hipLaunchKernelGGL(k_gen_operands, dim3(3), dim3(256), 0, s, ops, axis_len[0], axis_len[1], axis_len[2]);
   if (opsm)
     hipLaunchKernelGGL(k_gen_operands_mixed, dim3(3), dim3(192), 0, s, opsm, axis_len[0], axis_len[1], axis_len[2]);
+  return hipGetLastError();
+}
+
+// SplitMix64 stream on the device: element i is output i of the generator seeded with `seed` (the same
+// values ffs_fill_uniform writes on the host), so timing runs see data like the caller's, not zeros.
+__global__ void k_fill_uniform(double* d, int64_t n, uint64_t seed) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    d[i] = (double)(z >> 11) * 0x1.0p-53;
+  }
+}
+
+hipError_t fill_uniform_device(double* d, int64_t n, uint64_t seed, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_uniform, dim3(1024), dim3(256), 0, s, d, n, seed);
   return hipGetLastError();
 }
 

@@ -53,7 +53,8 @@ static void print_help() {
       "  --convention C       ref (default, out[p] = sum_a w[a] in[p-a]) | centred (out[p] = sum_o w[o+r] in[p+o])\n"
       "  --algorithm A        fft (default: tiled FFT stencil) | direct (FP64 stencil sum, temporal blocking) |\n"
       "                       auto (direct where offered)\n"
-      "  --mapping M          xcd (default: each XCD takes a contiguous band of tiles) | linear\n"
+      "  --mapping M          linear (default: hardware workgroup order) | xcd (each XCD takes a contiguous\n"
+      "                       band of tiles)\n"
       "  --seed S             seed of the weights and the initial grid (default 0x5EED0000)\n"
       "  --fuse t             fused time depth (default automatic)\n"
       "  --no-graph           plain launches instead of a hipGraph replay\n"
@@ -80,7 +81,7 @@ static bool parse_i64(const char* s, long long* v) {
 
 int main(int argc, char** argv) {
   std::vector<std::string> pos;
-  std::string custom, prec = "fp64", conv = "ref", algo = "fft", mapping = "xcd";
+  std::string custom, prec = "fp64", conv = "ref", algo = "fft", mapping = "linear";
   unsigned long long seed = 0x5EED0000ULL;
   int fuse = 0, use_graph = 1;
   bool tune = false;

@@ -198,3 +198,11 @@ def test_run_captured_in_caller_graph(ffs, oracle, torch_cuda, algorithm, precis
         torch.cuda.synchronize()
         assert torch.equal(out, eager), (k, float(scratch.sum()))
     plan.close()
+
+
+def test_fill_uniform_device_matches_host(ffs, oracle, torch_cuda):
+    torch = torch_cuda
+    d = torch.empty(100003, dtype=torch.float64, device="cuda")
+    ffs.fill_uniform_device(0x5EED0002, d)
+    torch.cuda.synchronize()
+    assert np.array_equal(d.cpu().numpy(), ffs.fill_uniform(0x5EED0002, d.numel()))
