@@ -281,9 +281,8 @@ def main():
 
     main_form = forms[0] if forms[0] in results else next(iter(results))
     r = results[main_form]
-    traffic = _traffic(args.workload, main_form)
-    if traffic is not None:
-        r["roofline"]["traffic"] = traffic
+    for form, res in results.items():
+        res["roofline"]["traffic"] = _traffic(args.workload, form)
     line = {
         "metric": "GStencils/s (points x steps / s)",
         "value": r["value"],
