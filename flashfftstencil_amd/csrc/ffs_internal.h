@@ -56,7 +56,10 @@ struct TileArgs {
   int32_t scale_exp;  // FP16 form: the tile pair is scaled so that its largest fluctuation is below 2^scale_exp
   double dc_gain;  // K(0)^t: gain of the constant mode (mixed forms add mean * gain back in FP64)
   int32_t linear_map;  // 1: workgroups take tiles in hardware order; 0: XCD-contiguous bands
-  int32_t pad_;
+  // 32-point mixed forms: 1 when the valid region is exactly window block (1, 1), i.e. h0 = h1 = 16 and
+  // V0, V1 <= 16.  The two inverse stages then compute only the outputs of that block (2 and 1 of 4
+  // output blocks); every value they keep is produced by the same instructions as in the full stages.
+  int32_t out16;
 };
 
 // Direct (non-FFT) FP64 kernels with temporal blocking (ffs_direct.hip).

@@ -327,6 +327,16 @@ typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 #define FFS_MFMA_BF16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16((a), (b), (c), 0, 0, 0)
 #define FFS_MFMA_F32(a, b, c) __builtin_amdgcn_mfma_f32_16x16x4f32((a), (b), (c), 0, 0, 0)
 
+// Input spacing of the 32-point stages.  With the inverse stages cut to the valid output block, the
+// compiler's schedule put the split of a stage's input 8-9 wait states after the MFMAs that produced it
+// (its own minimum for v_mfma_f32_16x16x32_{f16,bf16} on gfx950), and on MI355X that code returned
+// results that differ from the full stages by up to 1e-5 (FP16 form; 1e-7 expected).  With the four input
+// blocks of each split first passed through a 16-wait-state step, the reduced stages are bit-identical
+// to the full ones.  The step ties only the registers it names; it is not volatile.
+__device__ __forceinline__ void mfma_settle(f4& a, f4& b, f4& c, f4& d) {
+  asm("s_nop 7\n\ts_nop 7" : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
+}
+
 enum { PREC_BF16 = 1, PREC_F32 = 2, PREC_BF16X2 = 3, PREC_F16 = 4 };
 
 template <int NTERMS>
@@ -648,16 +658,20 @@ __device__ __forceinline__ void load_f32ops(const u4* __restrict__ sF, int arr, 
   for (int t = 0; t < 3; t++) F[t] = as_bf8(sF[((arr * 2 + Kb) * 3 + t) * kLanes + lane]);
 }
 
-template <bool INV, int NTERMS>
+// Output blocks (I, Kb) with I in [I0, 2) and Kb in [K0, 2): all four for the forward stages; for the
+// inverse stages of an out16 tile Kb = 1 only (the valid window rows 16..31), then (1, 1) only.
+template <bool INV, int NTERMS, int I0 = 0, int K0 = 0>
 __device__ __forceinline__ void stage32(const Blk4& zr, const Blk4& zi, const u4* __restrict__ sF, int lane, Blk4& outr,
                                         Blk4& outi) {
 #pragma unroll
-  for (int I = 0; I < 2; I++) {
+  for (int I = I0; I < 2; I++) {
     bf8 rh, rm, rl, ih, im, il;
-    split8<NTERMS>(zr.b[I], zr.b[2 + I], rh, rm, rl);
-    split8<NTERMS>(zi.b[I], zi.b[2 + I], ih, im, il);
+    f4 x0 = zr.b[I], x1 = zr.b[2 + I], y0 = zi.b[I], y1 = zi.b[2 + I];
+    mfma_settle(x0, x1, y0, y1);
+    split8<NTERMS>(x0, x1, rh, rm, rl);
+    split8<NTERMS>(y0, y1, ih, im, il);
 #pragma unroll
-    for (int Kb = 0; Kb < 2; Kb++) {
+    for (int Kb = K0; Kb < 2; Kb++) {
       bf8 Fr[3], Fi[3], Fn[3];
       load_f32ops(sF, 0, Kb, lane, Fr);
       load_f32ops(sF, 1, Kb, lane, Fi);
@@ -736,9 +750,27 @@ __device__ __forceinline__ void store32(const TileArgs& a, const PairCoord& pc, 
   }
 }
 
+// Valid region of an out16 tile pair: output block 3 = window rows and columns 16..31.
+__device__ __forceinline__ void store16(const TileArgs& a, const PairCoord& pc, int q, int j, const f4& yr,
+                                        const f4& yi, double addA, double addB, double scale = 1.0) {
+  const int64_t cA = pc.orgA + j, cB = pc.orgB + j;  // window column 16 + j
+  const bool okA = j < a.V1 && cA < a.n1;
+  const bool okB = j < a.V1 && pc.hasB && cB < a.n1;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int row = 4 * q + v;  // window row 16 + row
+    const int64_t y = pc.r0 + row;
+    if (row < a.V0 && y < a.n0) {
+      double* drow = a.dst + y * a.n1;
+      if (okA) drow[cA] = (double)yr[v] * scale + addA;
+      if (okB) drow[cB] = (double)yi[v] * scale + addB;
+    }
+  }
+}
+
 // One tile pair of the 32-point family.  The spectrum comes from LDS (single-grid launch) or from the
 // grid's FP64 table in global memory (batch launch, where every grid has its own stencil and depth).
-template <int NTERMS, bool H_FROM_LDS>
+template <int NTERMS, bool H_FROM_LDS, bool OUT16>
 __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const u4* __restrict__ sF,
                                             const f2* __restrict__ sH, int lane, int q, int j) {
   const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
@@ -771,9 +803,15 @@ __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const
       zr.b[b][v] = yr;
       zi.b[b][v] = yi;
     }
-  stage32<true, NTERMS>(zr, zi, sF, lane, orr, oi);
-  stage32<true, NTERMS>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
-  store32(a, pc, q, j, zr, zi, cA * a.dc_gain, cB * a.dc_gain);
+  if (OUT16) {
+    stage32<true, NTERMS, 0, 1>(zr, zi, sF, lane, orr, oi);  // blocks 1 and 3
+    stage32<true, NTERMS, 1, 1>(orr, oi, sF, lane, zr, zi);  // block 3 = the valid 16 x 16
+    store16(a, pc, q, j, zr.b[3], zi.b[3], cA * a.dc_gain, cB * a.dc_gain);
+  } else {
+    stage32<true, NTERMS>(zr, zi, sF, lane, orr, oi);
+    stage32<true, NTERMS>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
+    store32(a, pc, q, j, zr, zi, cA * a.dc_gain, cB * a.dc_gain);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -821,16 +859,18 @@ __device__ __forceinline__ void split8_f16(const f4& x0, const f4& x1, h8& h, h8
   l = as_h8(lw);
 }
 
-template <bool INV>
+template <bool INV, int I0 = 0, int K0 = 0>
 __device__ __forceinline__ void stage32_f16(const Blk4& zr, const Blk4& zi, const u4* __restrict__ sF, int lane,
                                             Blk4& outr, Blk4& outi) {
 #pragma unroll
-  for (int I = 0; I < 2; I++) {
+  for (int I = I0; I < 2; I++) {
     h8 rh, rl, ih, il;
-    split8_f16(zr.b[I], zr.b[2 + I], rh, rl);
-    split8_f16(zi.b[I], zi.b[2 + I], ih, il);
+    f4 x0 = zr.b[I], x1 = zr.b[2 + I], y0 = zi.b[I], y1 = zi.b[2 + I];
+    mfma_settle(x0, x1, y0, y1);
+    split8_f16(x0, x1, rh, rl);
+    split8_f16(y0, y1, ih, il);
 #pragma unroll
-    for (int Kb = 0; Kb < 2; Kb++) {
+    for (int Kb = K0; Kb < 2; Kb++) {
       h8 Fr[2], Fi[2], Fn[2];
 #pragma unroll
       for (int t = 0; t < 2; t++) {
@@ -860,7 +900,7 @@ __device__ __forceinline__ void stage32_f16(const Blk4& zr, const Blk4& zi, cons
   }
 }
 
-template <bool H_FROM_LDS>
+template <bool H_FROM_LDS, bool OUT16>
 __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, const u4* __restrict__ sF,
                                                 const f2* __restrict__ sH, int lane, int q, int j) {
   const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
@@ -905,9 +945,15 @@ __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, c
       zr.b[b][v] = yr;
       zi.b[b][v] = yi;
     }
-  stage32_f16<true>(zr, zi, sF, lane, orr, oi);
-  stage32_f16<true>(orr, oi, sF, lane, zr, zi);
-  store32(a, pc, q, j, zr, zi, cA * a.dc_gain, cB * a.dc_gain, ldexp(1.0, -e));
+  if (OUT16) {
+    stage32_f16<true, 0, 1>(zr, zi, sF, lane, orr, oi);
+    stage32_f16<true, 1, 1>(orr, oi, sF, lane, zr, zi);
+    store16(a, pc, q, j, zr.b[3], zi.b[3], cA * a.dc_gain, cB * a.dc_gain, ldexp(1.0, -e));
+  } else {
+    stage32_f16<true>(zr, zi, sF, lane, orr, oi);
+    stage32_f16<true>(orr, oi, sF, lane, zr, zi);
+    store32(a, pc, q, j, zr, zi, cA * a.dc_gain, cB * a.dc_gain, ldexp(1.0, -e));
+  }
 }
 
 __device__ __forceinline__ void stage_ops32(u4* sF, const uint32_t* __restrict__ opsm) {
@@ -920,7 +966,7 @@ __device__ __forceinline__ void stage_ops32(u4* sF, const uint32_t* __restrict__
   }
 }
 
-template <int NTERMS>
+template <int NTERMS, bool OUT16>
 __global__ __launch_bounds__(256, 3) void k_tile32_bf16(TileArgs a) {
   __shared__ u4 sF[3 * 2 * 3 * kLanes];  // B operands, 18 KB
   __shared__ f2 sH[4 * 4 * kLanes];      // spectrum [block][reg][lane] (re, im), 8 KB
@@ -931,7 +977,7 @@ __global__ __launch_bounds__(256, 3) void k_tile32_bf16(TileArgs a) {
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair<NTERMS, true>(a, tp, sF, sH, lane, q, j);
+  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair<NTERMS, true, OUT16>(a, tp, sF, sH, lane, q, j);
 }
 
 // Multi-grid launch: one fused block of every grid still running, as one persistent launch over the
@@ -959,11 +1005,15 @@ __global__ __launch_bounds__(256, 3) void k_batch32_bf16(const TileArgs* __restr
       if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
     const TileArgs a = items[lo];
-    tile32_pair<NTERMS, false>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+    if (a.out16)
+      tile32_pair<NTERMS, false, true>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+    else
+      tile32_pair<NTERMS, false, false>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
   }
 }
 
 // The FP16 form of the two kernels above (same structure, FP16 operand pairs).
+template <bool OUT16>
 __global__ __launch_bounds__(256, 3) void k_tile32_f16(TileArgs a) {
   __shared__ u4 sF[3 * 2 * 3 * kLanes];
   __shared__ f2 sH[4 * 4 * kLanes];
@@ -974,7 +1024,7 @@ __global__ __launch_bounds__(256, 3) void k_tile32_f16(TileArgs a) {
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair_f16<true>(a, tp, sF, sH, lane, q, j);
+  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair_f16<true, OUT16>(a, tp, sF, sH, lane, q, j);
 }
 
 __global__ __launch_bounds__(256, 3) void k_batch32_f16(const TileArgs* __restrict__ items,
@@ -994,7 +1044,10 @@ __global__ __launch_bounds__(256, 3) void k_batch32_f16(const TileArgs* __restri
       if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
     const TileArgs a = items[lo];
-    tile32_pair_f16<false>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+    if (a.out16)
+      tile32_pair_f16<false, true>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+    else
+      tile32_pair_f16<false, false>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
   }
 }
 
@@ -1508,12 +1561,18 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
     else
       hipLaunchKernelGGL(k_tile16_f64<KIND_1DPFA>, g, b, 0, s, a);
   } else if (kind == KIND_2D32) {
-    if (a.precision == PREC_BF16)
-      hipLaunchKernelGGL(k_tile32_bf16<3>, g, b, 0, s, a);
+    if (a.precision == PREC_BF16 && a.out16)
+      hipLaunchKernelGGL((k_tile32_bf16<3, true>), g, b, 0, s, a);
+    else if (a.precision == PREC_BF16)
+      hipLaunchKernelGGL((k_tile32_bf16<3, false>), g, b, 0, s, a);
+    else if (a.precision == PREC_BF16X2 && a.out16)
+      hipLaunchKernelGGL((k_tile32_bf16<2, true>), g, b, 0, s, a);
     else if (a.precision == PREC_BF16X2)
-      hipLaunchKernelGGL(k_tile32_bf16<2>, g, b, 0, s, a);
+      hipLaunchKernelGGL((k_tile32_bf16<2, false>), g, b, 0, s, a);
+    else if (a.precision == PREC_F16 && a.out16)
+      hipLaunchKernelGGL(k_tile32_f16<true>, g, b, 0, s, a);
     else if (a.precision == PREC_F16)
-      hipLaunchKernelGGL(k_tile32_f16, g, b, 0, s, a);
+      hipLaunchKernelGGL(k_tile32_f16<false>, g, b, 0, s, a);
     else
       return hipErrorInvalidValue;
   } else {

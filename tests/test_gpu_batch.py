@@ -46,6 +46,35 @@ def test_batch_matches_oracle_and_single_plans(ffs, oracle, torch_cuda, form):
     batch.close()
 
 
+@pytest.mark.parametrize("form", ["bf16", "f16"])
+def test_batch_mixes_valid_widths(ffs, oracle, torch_cuda, form):
+    # one launch holds tile pairs with V <= 16 (reduced inverse stages) and V > 16 (full ones)
+    torch = torch_cuda
+    specs = []
+    for i, (r, fuse, sizes, steps) in enumerate([(2, 4, (70, 45), 9), (2, 3, (33, 61), 7), (1, 8, (50, 50), 17),
+                                                 (1, 5, (40, 29), 11), (3, 3, (64, 64), 6)]):
+        rng = np.random.default_rng(300 + i)
+        w = rng.random((2 * r + 1) ** 2)
+        w /= w.sum()
+        specs.append(dict(radius=r, weights=w, sizes=sizes, steps=steps, fuse=fuse))
+    xs = [oracle.uniform(500 + i, s["sizes"]) for i, s in enumerate(specs)]
+    d_in = [torch.from_numpy(x).cuda() for x in xs]
+    d_out = [torch.empty_like(d) for d in d_in]
+    batch = ffs.Batch(specs, precision=form)
+    batch.run(d_in, d_out)
+    torch.cuda.synchronize()
+    for i, s in enumerate(specs):
+        ref = oracle.sweep(xs[i], s["radius"], s["weights"], s["steps"])
+        assert oracle.errors(d_out[i].cpu().numpy(), ref)[1] <= 1e-6, (i, batch.info(i))
+        plan = ffs.Plan(2, s["sizes"], s["weights"], s["radius"], s["steps"], precision=form, fuse=s["fuse"])
+        single = torch.empty_like(d_in[i])
+        plan.run(d_in[i], single)
+        torch.cuda.synchronize()
+        plan.close()
+        assert torch.equal(single, d_out[i]), i
+    batch.close()
+
+
 def test_batch_rejects_mixed_precisions(ffs):
     _, r, w, _ = ffs.named_stencil("Box2D9P")
     specs = [dict(radius=r, weights=w, sizes=(32, 32), steps=2, precision="bf16"),

@@ -59,6 +59,25 @@ def test_2d_mixed_fuse_convention(ffs, oracle, torch_cuda, form, convention, fus
     assert oracle.errors(got, ref)[1] <= TOL_MIXED
 
 
+# Valid widths on both sides of 16 in the 32-point family: V = 32 - 2 r t <= 16 takes the reduced inverse
+# stages (shifted spectrum, one output block), V > 16 the full ones.  Steps 2 t + 1 add a tail block of
+# depth 1 whose shift differs from the main block's.
+@pytest.mark.parametrize("form", ["bf16", "bf16x2", "f16"])
+@pytest.mark.parametrize("convention", ["ref", "centred"])
+@pytest.mark.parametrize("radius,fuse", [(1, 7), (1, 8), (1, 9), (2, 3), (2, 4), (2, 5), (3, 3), (3, 5)])
+def test_2d_mixed_valid_16_boundary(ffs, oracle, torch_cuda, form, convention, radius, fuse):
+    rng = np.random.default_rng(100 * radius + fuse)
+    w = rng.random((2 * radius + 1) ** 2)
+    w /= w.sum()
+    x = oracle.uniform(91 + fuse, (53, 71))
+    steps = 2 * fuse + 1
+    got, info = _run(ffs, torch_cuda, x, radius, w, steps, precision=form, convention=convention, fuse=fuse)
+    assert info["tile"][:2] == [32, 32] and info["valid"][0] == 32 - 2 * radius * fuse
+    ref = oracle.sweep(x, radius, w, steps, convention=0 if convention == "ref" else 1)
+    tol = 2e-5 if form == "bf16x2" else TOL_MIXED
+    assert oracle.errors(got, ref)[1] <= tol, (form, convention, radius, fuse, info)
+
+
 @pytest.mark.parametrize("form", ["bf16", "f32", "f16"])
 def test_2d_mixed_gain_not_one(ffs, oracle, torch_cuda, form):
     # weights summing to 1.3: the constant mode is carried in FP64 with gain (sum w)^t
