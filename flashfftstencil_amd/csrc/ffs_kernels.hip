@@ -1012,9 +1012,10 @@ __global__ __launch_bounds__(256, 3) void k_batch32_bf16(const TileArgs* __restr
   }
 }
 
-// The FP16 form of the two kernels above (same structure, FP16 operand pairs).
+// The FP16 form of the two kernels above (same structure, FP16 operand pairs).  The out16 variant fits
+// 128 VGPRs and runs 4 waves per SIMD (measured 5% faster than 3).
 template <bool OUT16>
-__global__ __launch_bounds__(256, 3) void k_tile32_f16(TileArgs a) {
+__global__ __launch_bounds__(256, OUT16 ? 4 : 3) void k_tile32_f16(TileArgs a) {
   __shared__ u4 sF[3 * 2 * 3 * kLanes];
   __shared__ f2 sH[4 * 4 * kLanes];
   stage_ops32(sF, a.opsm);
@@ -1537,9 +1538,11 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
   }
   const int64_t waves_needed = a.pairs;
   int64_t blocks = (waves_needed + 3) / 4;
-  // 8 workgroups (32 waves) per CU at most, grid-stride beyond; the 32-point kernel stages its tables
-  // in LDS per workgroup and runs 3 waves per SIMD, so it launches exactly the resident workgroups.
-  const int64_t cap = (int64_t)num_cus * (kind == KIND_2D32 ? 3 : 8);
+  // 8 workgroups (32 waves) per CU at most, grid-stride beyond; the 32-point kernels stage their tables
+  // in LDS per workgroup and run 3 waves per SIMD (4 for the FP16 out16 kernel), so they launch exactly
+  // the resident workgroups.
+  const int64_t per_cu32 = (a.precision == PREC_F16 && a.out16) ? 4 : 3;
+  const int64_t cap = (int64_t)num_cus * (kind == KIND_2D32 ? per_cu32 : 8);
   if (blocks > cap) blocks = cap;
   const dim3 g((unsigned)blocks), b(256);
   if (kind == KIND_2D16) {
