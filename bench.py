@@ -36,7 +36,8 @@ HBM_PEAK = 8.0e12
 WORKLOADS = {
     "box2d25p": dict(shape="Box2D25P", sizes=(8192, 8192), T=256, seed=0x5EED0002,
                      forms=("f16", "bf16", "fp64", "direct")),
-    "box2d9p": dict(shape="Box2D9P", sizes=(8192, 8192), T=128, seed=0x5EED0001, forms=("fp64", "direct")),
+    "box2d9p": dict(shape="Box2D9P", sizes=(8192, 8192), T=128, seed=0x5EED0001,
+                    forms=("f16", "bf16", "fp64", "direct")),
     "heat1d": dict(shape="Heat-1D", sizes=(1048576,), T=64, seed=0x5EED0000, forms=("fp64", "direct")),
     "box3d27p": dict(shape="Box3D27P", sizes=(512, 512, 512), T=32, seed=0x5EED0003,
                      forms=("bf16", "f32", "fp64", "direct"), init="sinusoids"),
