@@ -245,10 +245,12 @@ ffs_status choose_config(ffs_plan* pl) {
   } else if (p.dims == 2) {
     const bool bf = p.precision == FFS_PREC_MIXED_BF16 || p.precision == FFS_PREC_MIXED_BF16X2;
     const bool f16 = p.precision == FFS_PREC_MIXED_F16;
+    const bool fp64 = p.precision == FFS_PREC_FP64;
     const int P = p.tile != 0 ? p.tile : (bf || f16 ? 32 : 16);
-    if (!((P == 16 && !f16) || (P == 32 && (bf || f16))))
+    if (!((P == 16 && !f16) || (P == 32 && (bf || f16 || fp64))))
       return fail(FFS_ERR_UNSUPPORTED,
-                  "2D tile %d not available for this precision (16 or 32 with the BF16 forms, 32 with FP16, else 16)", P);
+                  "2D tile %d not available for this precision (16 or 32 with FP64 and the BF16 forms, 32 with FP16, "
+                  "16 with FP32)", P);
     pl->kind = P == 32 ? ffsk::KIND_2D32 : ffsk::KIND_2D16;
     pl->P[0] = pl->P[1] = P;
     pl->L[0] = pl->L[1] = P;
@@ -257,43 +259,57 @@ ffs_status choose_config(ffs_plan* pl) {
     pl->kind = ffsk::KIND_3D16;
     pl->P[0] = pl->P[1] = pl->P[2] = 16;
   }
-  const int Pmain = p.dims == 1 ? 240 : pl->P[0];
   const int vmin = p.dims == 1 ? 16 : 2;
-  // largest admissible depth: V = P - 2 r t >= vmin
-  int64_t tmax = r == 0 ? (int64_t)1 << 40 : (Pmain - vmin) / (2 * r);
-  if (tmax < 1) return fail(FFS_ERR_UNSUPPORTED, "radius %d leaves no valid region in a %d-wide tile", r, Pmain);
-  int64_t tbest;
-  if (p.fuse > 0) {
-    if (p.fuse > tmax)
-      return fail(FFS_ERR_UNSUPPORTED, "fuse %d too deep for radius %d and tile %d (max %lld)", p.fuse, r, Pmain,
-                  (long long)tmax);
-    tbest = p.fuse;
-  } else {
-    // Maximise valid point-steps per unit time of one tile pair: 2 V^d t / max(compute, hbm), where the
-    // compute cost of a pair is fixed by the tile family and precision (cycles per SIMD, measured on
-    // MI355X) and the HBM cost is 16 B per valid point at ~3.26 B/cycle/SIMD (8 TB/s over 1024 SIMDs
-    // at 2.4 GHz).
-    const int dd = p.dims == 1 ? 1 : p.dims;
+  const int dd = p.dims == 1 ? 1 : p.dims;
+  const int hi_step = p.convention == FFS_CONV_REF ? 2 * r : r;
+  // Maximise valid point-steps per unit time of one tile pair: 2 V^d t / max(compute, hbm), where the
+  // compute cost of a pair is fixed by the tile family and precision (cycles per SIMD, measured on
+  // MI355X) and the HBM cost is 16 B per valid point at ~3.26 B/cycle/SIMD (8 TB/s over 1024 SIMDs at
+  // 2.4 GHz).  Returns the score of the best (or the forced) depth of a family, -1 when none fits.
+  auto search = [&](int kind, int Pm, int64_t& t_out) -> double {
+    const int64_t tmax = r == 0 ? (int64_t)1 << 40 : (Pm - vmin) / (2 * r);
+    if (tmax < 1 || (p.fuse > 0 && p.fuse > tmax)) return -1.0;
     double compute = p.precision == FFS_PREC_FP64 ? 4700.0 : (p.precision == FFS_PREC_MIXED_F32 ? 2600.0 : 2500.0);
-    if (pl->kind == ffsk::KIND_2D32)
-      compute = p.precision == FFS_PREC_MIXED_BF16 ? 12000.0 : (p.precision == FFS_PREC_MIXED_F16 ? 3000.0 : 9000.0);
+    if (kind == ffsk::KIND_2D32)
+      compute = p.precision == FFS_PREC_MIXED_BF16 ? 12000.0
+                : p.precision == FFS_PREC_MIXED_F16 ? 3000.0
+                : p.precision == FFS_PREC_FP64     ? 31000.0  // 384 FP64 MFMAs per pair
+                                                   : 9000.0;
+    // 32-point family with the valid region on window block (1, 1): reduced inverse stages, measured at
+    // 0.63 (FP16), 0.70 (BF16) and 0.74 (FP64) of a full tile pair on MI355X
+    const double cut = p.precision == FFS_PREC_FP64 ? 0.74 : 0.65;
     const double hbm_per_point = 2.0 * 16.0 / 3.26;
     double best = -1;
-    tbest = 1;
-    const int64_t tcap = std::min<int64_t>(tmax, 4096);
-    const int hi_step = p.convention == FFS_CONV_REF ? 2 * r : r;
-    for (int64_t t = 1; t <= tcap; t++) {
-      const double Vd = std::pow(Pmain - 2.0 * r * t, dd);
-      // 32-point family with the valid region on window block (1, 1): reduced inverse stages, measured
-      // at 0.63 (FP16) and 0.70 (BF16) of a full tile pair on MI355X
-      const bool out16 = pl->kind == ffsk::KIND_2D32 && hi_step * t == 16;
-      const double c = compute * (p.dims == 3 ? 16.0 : 1.0) * (out16 ? 0.65 : 1.0);
+    const int64_t t0 = p.fuse > 0 ? p.fuse : 1, t1 = p.fuse > 0 ? p.fuse : std::min<int64_t>(tmax, 4096);
+    for (int64_t t = t0; t <= t1; t++) {
+      const double Vd = std::pow(Pm - 2.0 * r * t, dd);
+      const bool out16 = kind == ffsk::KIND_2D32 && hi_step * t == 16;
+      const double c = compute * (p.dims == 3 ? 16.0 : 1.0) * (out16 ? cut : 1.0);
       const double score = Vd * t / std::max(c, hbm_per_point * Vd);
       if (score > best * (1 + 1e-9)) {
         best = score;
-        tbest = t;
+        t_out = t;
       }
     }
+    return best;
+  };
+  int64_t tbest = 1;
+  double score = search(pl->kind, p.dims == 1 ? 240 : pl->P[0], tbest);
+  if (p.dims == 2 && p.precision == FFS_PREC_FP64 && p.tile == 0) {  // FP64: 16- or 32-point tiles
+    int64_t t32 = 1;
+    const double s32 = search(ffsk::KIND_2D32, 32, t32);
+    if (s32 > score * (1 + 1e-9)) {
+      pl->kind = ffsk::KIND_2D32;
+      pl->P[0] = pl->P[1] = pl->L[0] = pl->L[1] = 32;
+      score = s32;
+      tbest = t32;
+    }
+  }
+  const int Pmain = p.dims == 1 ? 240 : pl->P[0];
+  if (score < 0) {
+    if (p.fuse > 0)
+      return fail(FFS_ERR_UNSUPPORTED, "fuse %d too deep for radius %d and tile %d", p.fuse, r, Pmain);
+    return fail(FFS_ERR_UNSUPPORTED, "radius %d leaves no valid region in a %d-wide tile", r, Pmain);
   }
   set_blocks(pl, tbest, p.fuse > 0);
   const int hi = p.convention == FFS_CONV_REF ? 2 * r : r;  // largest tap offset of one step
@@ -598,7 +614,7 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
     return FFS_OK;
   }
   pl->hlen = ffsk::spectrum_len(pl->kind);
-  chk(hipMalloc(&pl->d_ops, sizeof(double) * ffsk::kOpsLen));
+  chk(hipMalloc(&pl->d_ops, sizeof(double) * std::max(ffsk::kOpsLen, ffsk::kOps32F64)));
   if (p->precision != FFS_PREC_FP64)
     chk(hipMalloc(&pl->d_opsm, sizeof(uint32_t) * std::max(ffsk::kOpsMixedLen, ffsk::kOps32Words)));
   chk(hipMalloc(&pl->d_H, sizeof(double) * 4 * pl->hlen));
@@ -611,7 +627,9 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
   }
   chk(hipMemcpy(pl->d_taps, pl->weights.data(), sizeof(double) * ntaps, hipMemcpyHostToDevice));
   const int layout = p->precision == FFS_PREC_FP64 ? 0 : 1;
-  if (pl->kind == ffsk::KIND_2D32)
+  if (pl->kind == ffsk::KIND_2D32 && p->precision == FFS_PREC_FP64)
+    chk(ffsk::gen_operands32_f64(pl->d_ops, nullptr));
+  else if (pl->kind == ffsk::KIND_2D32)
     chk(ffsk::gen_operands32(pl->d_opsm, p->precision == FFS_PREC_MIXED_F16, nullptr));
   else
     chk(ffsk::gen_operands(pl->d_ops, pl->d_opsm, pl->L, nullptr));
@@ -752,7 +770,9 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   double per_pair = 2.0 * dd * per_cplx * std::pow(16.0, dd - 2);
   // 32-point family: output blocks over the 4 stages (4 + 4 + 4 + 4, or 4 + 4 + 2 + 1 for an out16 main
   // block) x 4 real GEMMs x (6, 4 or 3) K=32 MFMAs
-  if (pl->kind == ffsk::KIND_2D32)
+  if (pl->kind == ffsk::KIND_2D32 && p.precision == FFS_PREC_FP64)  // x 3 Gauss GEMMs x 8 K=4 chunks
+    per_pair = (out16_block(pl, false) ? 11.0 : 16.0) * 3.0 * 8.0 * 2.0 * 16 * 16 * 4;
+  else if (pl->kind == ffsk::KIND_2D32)
     per_pair = (out16_block(pl, false) ? 11.0 : 16.0) * 4.0 *
                (p.precision == FFS_PREC_MIXED_BF16 ? 6.0 : (p.precision == FFS_PREC_MIXED_F16 ? 3.0 : 4.0)) *
                2.0 * 16 * 16 * 32;
@@ -910,6 +930,7 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
     else if (p->dims == 3) tiles = {16};
     else if (p->precision == FFS_PREC_MIXED_BF16 || p->precision == FFS_PREC_MIXED_BF16X2) tiles = {16, 32};
     else if (p->precision == FFS_PREC_MIXED_F16) tiles = {32};
+    else if (p->precision == FFS_PREC_FP64) tiles = {16, 32};
     else tiles = {16};
     for (int tile : tiles) {
       const int vmin = p->dims == 1 ? 16 : 2;

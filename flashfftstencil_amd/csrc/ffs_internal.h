@@ -10,7 +10,7 @@ enum TileKind : int32_t {
   KIND_2D16 = 0,   // 2D, 16x16 window
   KIND_1DPFA = 1,  // 1D, 240-sample window as a 15x16 prime-factor (Good-Thomas) matrix
   KIND_3D16 = 2,   // 3D, 16x16x16 window (one workgroup of 4 waves per tile pair)
-  KIND_2D32 = 3,   // 2D, 32x32 window (mixed BF16 form: the contraction fills K = 32 of the MFMA)
+  KIND_2D32 = 3,   // 2D, 32x32 window (mixed forms: the contraction fills K = 32 of the MFMA; FP64: 8 chunks)
 };
 
 // Operand-table layout in doubles (one table per plan, generated on the GPU in FP64):
@@ -40,6 +40,10 @@ inline int64_t spectrum_len(int kind) {  // complex values per spectrum (one dep
 // the 3-term BF16 split, or h, l (and a zero term) of the 2-term FP16 split for the FP16 form;
 // element e of lane (j + 16q) is row 16 (e >> 2) + 4q + (e & 3), column 16 Kb + j of that term.
 constexpr int kOps32Words = 3 * 2 * 3 * 4 * kLanes;  // 4608
+
+// 32-point FP64 B operands (KIND_2D32, FP64 form): [R][chunk v][column block Kb][arr: R, S, D][lane], lane
+// (n + 16k) holding entry (row 16 R + 4 v + k, column 16 Kb + n) of Fr, Fr + Fi or Fi - Fr.
+constexpr int kOps32F64 = 2 * 4 * 2 * 3 * kLanes;  // 3072 doubles
 
 struct TileArgs {
   const double* src;
@@ -91,6 +95,7 @@ hipError_t launch_direct(const DirectArgs& a, int dims, int radius, bool star, h
 // Launch helpers implemented in ffs_kernels.hip.
 hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hipStream_t s);
 hipError_t gen_operands32(uint32_t* ops32, bool f16, hipStream_t s);
+hipError_t gen_operands32_f64(double* ops, hipStream_t s);
 // d[i] = output i of SplitMix64(seed) as a double in [0, 1) (the host ffs_fill_uniform sequence)
 hipError_t fill_uniform_device(double* d, int64_t n, uint64_t seed, hipStream_t s);
 // layout 0: FP64 accumulator order (row freq q + 4v); layout 1: FP32 accumulator order (row freq 4q + v)

@@ -294,6 +294,152 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// 2D 32x32 tiles, FP64.  A 32x32 matrix is held as 4 blocks (R, C) of 16x16 in FP64-accumulator order:
+//   block b = 2R + C, lane (j + 16q), reg v  <->  M[16R + q + 4v][16C + j],
+// so register v of block (R, I) is the A operand of contraction rows 16R + 4v .. + 3 (lane (i + 16k)
+// holds M[16R + 4v + k][16I + i]) and a stage Out = Z^T F is, per output block (I, Kb), 8 chunks x 3
+// Gauss GEMMs = 24 v_mfma_f64_16x16x4f64 into three accumulators, as in the 16-point kernel.  With OUT16
+// (valid region = window block (1, 1)) the inverse stages compute only the outputs kept: blocks (I, 1),
+// then (1, 1).  DFT operands (24 KB) and the FP64 spectrum (16 KB) are staged in LDS per workgroup.
+// ---------------------------------------------------------------------------------------------
+struct Blk4d {
+  d4 b[4];
+};
+
+template <bool INV, int I0, int K0>
+__device__ __forceinline__ void stage32_f64(const Blk4d& zr, const Blk4d& zi, const double* __restrict__ sF, int lane,
+                                            Blk4d& outr, Blk4d& outi) {
+#pragma unroll
+  for (int I = I0; I < 2; I++) {
+    double sum[2][4];
+#pragma unroll
+    for (int R = 0; R < 2; R++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) sum[R][v] = zr.b[2 * R + I][v] + zi.b[2 * R + I][v];
+#pragma unroll
+    for (int Kb = K0; Kb < 2; Kb++) {
+      d4 k1 = {0.0, 0.0, 0.0, 0.0}, k2 = k1, k3 = k1;
+#pragma unroll
+      for (int R = 0; R < 2; R++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const double* f = sF + (((R * 4 + v) * 2 + Kb) * 3) * kLanes + lane;
+          const double fr = f[0], fs = f[kLanes], fd = f[2 * kLanes];
+          k1 = FFS_MFMA64(sum[R][v], fr, k1);
+          k2 = FFS_MFMA64(zr.b[2 * R + I][v], INV ? fs : fd, k2);
+          k3 = FFS_MFMA64(zi.b[2 * R + I][v], INV ? fd : fs, k3);
+        }
+      if (!INV) {
+        outr.b[2 * I + Kb] = k1 - k3;
+        outi.b[2 * I + Kb] = k1 + k2;
+      } else {
+        outr.b[2 * I + Kb] = k1 + k3;
+        outi.b[2 * I + Kb] = k1 - k2;
+      }
+    }
+  }
+}
+
+// Window values of the pair in block order: x.b[2R + C][v] = window[16R + q + 4v][16C + j].
+__device__ __forceinline__ void gather32_f64(const TileArgs& a, const PairCoord& pc, int q, int j, Blk4d& xa,
+                                             Blk4d& xb) {
+  const double* __restrict__ src = a.src;
+  const int64_t rw = pc.r0 - a.h0, cwA = pc.orgA - a.h1, cwB = pc.orgB - a.h1;
+  const bool interior = rw >= 0 && rw + 32 <= a.n0 && cwA >= 0 && (pc.hasB ? cwB : cwA) + 32 <= a.n1;
+  if (interior) {
+    const double* p = src + (rw + q) * a.n1 + cwA + j;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int64_t off = (int64_t)(16 * (b >> 1) + 4 * v) * a.n1 + 16 * (b & 1);
+        xa.b[b][v] = p[off];
+        xb.b[b][v] = pc.hasB ? p[off + a.V1] : 0.0;
+      }
+  } else {
+#pragma unroll
+    for (int C = 0; C < 2; C++) {
+      const int64_t cA = wrap_near(cwA + 16 * C + j, a.n1);
+      const int64_t cB = wrap_near(cwB + 16 * C + j, a.n1);
+#pragma unroll
+      for (int R = 0; R < 2; R++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int64_t row = wrap_near(rw + 16 * R + q + 4 * v, a.n0) * a.n1;
+          xa.b[2 * R + C][v] = src[row + cA];
+          xb.b[2 * R + C][v] = pc.hasB ? src[row + cB] : 0.0;
+        }
+    }
+  }
+}
+
+// Store the valid outputs of blocks (R, C) with R >= R0, C >= C0 (OUT16: only (1, 1)).
+template <int R0, int C0>
+__device__ __forceinline__ void store32_f64(const TileArgs& a, const PairCoord& pc, int q, int j, const Blk4d& yr,
+                                            const Blk4d& yi) {
+#pragma unroll
+  for (int C = C0; C < 2; C++) {
+    const int wc = 16 * C + j;
+    const int64_t cA = pc.orgA + wc - a.h1, cB = pc.orgB + wc - a.h1;
+    const bool colok = wc >= a.h1 && wc < a.h1 + a.V1;
+    const bool okA = colok && cA < a.n1;
+    const bool okB = colok && pc.hasB && cB < a.n1;
+#pragma unroll
+    for (int R = R0; R < 2; R++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = 16 * R + q + 4 * v;
+        const int64_t y = pc.r0 + row - a.h0;
+        if (row >= a.h0 && row < a.h0 + a.V0 && y < a.n0) {
+          double* drow = a.dst + y * a.n1;
+          if (okA) drow[cA] = yr.b[2 * R + C][v];
+          if (okB) drow[cB] = yi.b[2 * R + C][v];
+        }
+      }
+  }
+}
+
+template <bool OUT16>
+__global__ __launch_bounds__(256, 2) void k_tile32_f64(TileArgs a) {
+  __shared__ double sF[kOps32F64];           // 24 KB
+  __shared__ double2 sH[4 * 4 * kLanes];     // spectrum [block][reg][lane] (re, im), 16 KB
+  for (int i = threadIdx.x; i < kOps32F64; i += blockDim.x) sF[i] = a.ops[i];
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) sH[i] = double2{a.H[i], a.H[1024 + i]};
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, j = lane & 15;
+  const int64_t W = (int64_t)gridDim.x * 4;
+  const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int64_t tp = gw; tp < a.pairs; tp += W) {
+    const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
+    Blk4d zr, zi, orr, oi;
+    gather32_f64(a, pc, q, j, zr, zi);
+    stage32_f64<false, 0, 0>(zr, zi, sF, lane, orr, oi);
+    stage32_f64<false, 0, 0>(orr, oi, sF, lane, zr, zi);  // Y[k0][k1], block order
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const double2 h = sH[(b * 4 + v) * kLanes + lane];
+        double yr = zr.b[b][v], yi = zi.b[b][v];
+        cmul(yr, yi, h.x, h.y);
+        zr.b[b][v] = yr;
+        zi.b[b][v] = yi;
+      }
+    if (OUT16) {
+      stage32_f64<true, 0, 1>(zr, zi, sF, lane, orr, oi);  // blocks 1 and 3
+      stage32_f64<true, 1, 1>(orr, oi, sF, lane, zr, zi);  // block 3 = the valid 16 x 16
+      store32_f64<1, 1>(a, pc, q, j, zr, zi);
+    } else {
+      stage32_f64<true, 0, 0>(zr, zi, sF, lane, orr, oi);
+      stage32_f64<true, 0, 0>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
+      store32_f64<0, 0>(a, pc, q, j, zr, zi);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Mixed-precision forms, 2D 16x16 and 1D prime-factor tiles.
 //
 // FP32 accumulator layout (v_mfma_f32_16x16x32_bf16 and v_mfma_f32_16x16x4f32, measured on gfx950):
@@ -1392,6 +1538,17 @@ __global__ void k_gen_operands32(uint32_t* t, int32_t f16) {
   }
 }
 
+// 32-point FP64 B operands (layout kOps32F64): one thread per entry.
+__global__ void k_gen_operands32_f64(double* t) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= kOps32F64) return;
+  const int lane = idx % kLanes, arr = (idx / kLanes) % 3, Kb = (idx / (3 * kLanes)) % 2;
+  const int v = (idx / (6 * kLanes)) % 4, R = idx / (24 * kLanes);
+  double fr, fi;
+  dft_entry(16 * R + 4 * v + (lane >> 4), 16 * Kb + (lane & 15), 32, fr, fi);
+  t[idx] = arr == 0 ? fr : (arr == 1 ? fr + fi : fi - fr);
+}
+
 // K(k) = sum_taps w[a] exp(-2 pi i sum_ax (d_ax(a) k_ax mod L_ax)/L_ax),  d = a (ref) or r - a (centred);
 // H = K^t / prod(L) by repeated squaring in complex FP64.  `kind` fixes which frequency sits at
 // entry e of the family layout.
@@ -1412,8 +1569,8 @@ __global__ void k_gen_spectrum(double* H, int kind, int layout, int32_t L0, int3
     v = (int)(e / kLanes);
   }
   int krow = layout == 0 ? q + 4 * v : 4 * q + v;  // frequency on the row axis of the family layout
-  if (kind == KIND_2D32) {  // block kb = 2 R + C: rows 16 R + 4q + v, columns 16 C + j
-    krow = 16 * (kb >> 1) + 4 * q + v;
+  if (kind == KIND_2D32) {  // block kb = 2 R + C: rows 16 R + (4q + v | q + 4v), columns 16 C + j
+    krow = 16 * (kb >> 1) + (layout == 0 ? q + 4 * v : 4 * q + v);
     j += 16 * (kb & 1);
   }
   double kr = 0.0, ki = 0.0;
@@ -1508,6 +1665,11 @@ hipError_t fill_uniform_device(double* d, int64_t n, uint64_t seed, hipStream_t 
   return hipGetLastError();
 }
 
+hipError_t gen_operands32_f64(double* ops, hipStream_t s) {
+  hipLaunchKernelGGL(k_gen_operands32_f64, dim3((kOps32F64 + 255) / 256), dim3(256), 0, s, ops);
+  return hipGetLastError();
+}
+
 hipError_t gen_operands32(uint32_t* ops32, bool f16, hipStream_t s) {
   hipLaunchKernelGGL(k_gen_operands32, dim3((3 * 2 * 3 * kLanes + 255) / 256), dim3(256), 0, s, ops32, (int32_t)f16);
   return hipGetLastError();
@@ -1541,7 +1703,7 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
   // 8 workgroups (32 waves) per CU at most, grid-stride beyond; the 32-point kernels stage their tables
   // in LDS per workgroup and run 3 waves per SIMD (4 for the FP16 out16 kernel), so they launch exactly
   // the resident workgroups.
-  const int64_t per_cu32 = (a.precision == PREC_F16 && a.out16) ? 4 : 3;
+  const int64_t per_cu32 = (a.precision == PREC_F16 && a.out16) ? 4 : (a.precision == 0 ? 2 : 3);
   const int64_t cap = (int64_t)num_cus * (kind == KIND_2D32 ? per_cu32 : 8);
   if (blocks > cap) blocks = cap;
   const dim3 g((unsigned)blocks), b(256);
@@ -1576,6 +1738,10 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
       hipLaunchKernelGGL(k_tile32_f16<true>, g, b, 0, s, a);
     else if (a.precision == PREC_F16)
       hipLaunchKernelGGL(k_tile32_f16<false>, g, b, 0, s, a);
+    else if (a.precision == 0 && a.out16)
+      hipLaunchKernelGGL(k_tile32_f64<true>, g, b, 0, s, a);
+    else if (a.precision == 0)
+      hipLaunchKernelGGL(k_tile32_f64<false>, g, b, 0, s, a);
     else
       return hipErrorInvalidValue;
   } else {

@@ -133,8 +133,10 @@ def test_plan_create_rejects_invalid(ffs, kw, needle):
 
 
 @pytest.mark.parametrize("kw,needle", [
-    (dict(tile=24), "tile"), (dict(tile=32, precision=0), "tile"), (dict(fuse=8), "fuse"),
-    (dict(radius=8, weights=np.full(17 * 17, 1 / 289.0)), "radius"),
+    (dict(tile=24), "tile"), (dict(tile=32, precision=2), "tile"), (dict(fuse=16), "fuse"),
+    (dict(fuse=8, tile=16), "fuse"),
+    (dict(radius=16, weights=np.full(33 * 33, 1 / 1089.0)), "radius"),
+    (dict(radius=8, tile=16, weights=np.full(17 * 17, 1 / 289.0)), "radius"),
     (dict(precision=4, tile=16), "tile"), (dict(precision=4, dims=1, sizes=[640]), "FP16"),
     (dict(precision=4, dims=3, sizes=[16, 16, 16], weights=np.full(27, 1 / 27.0)), "FP16"),
     (dict(algorithm=1, precision=1), "FP64"),
