@@ -97,7 +97,8 @@ typedef struct ffs_problem {
   int64_t steps;         /* T >= 0 */
   int32_t precision;     /* ffs_precision */
   int32_t convention;    /* ffs_convention */
-  int32_t tile;          /* DFT length per axis; 0 = automatic (2D/3D: 16; 1D: 240 = 16x15 prime-factor) */
+  int32_t tile;          /* DFT length per axis; 0 = automatic (1D: 240 = 16x15 prime-factor; 3D: 16; 2D: 32
+                            for the FP16/BF16 forms, 16 or 32 for FP64 by the depth cost model, 16 for FP32) */
   int32_t fuse;          /* fused depth t; 0 = automatic */
   int32_t use_graph;     /* 1 = replay the block loop from a hipGraph (chains of more than 4 launches;
                             shorter chains launch directly, which is cheaper), 0 = plain launches */
