@@ -41,7 +41,7 @@ WORKLOADS = {
     "heat1d": dict(shape="Heat-1D", sizes=(1048576,), T=64, seed=0x5EED0000, forms=("fp64", "direct")),
     "box3d27p": dict(shape="Box3D27P", sizes=(512, 512, 512), T=32, seed=0x5EED0003,
                      forms=("bf16", "f32", "fp64", "direct"), init="sinusoids"),
-    "batch96": dict(seed=0x5EED0004, forms=("f16", "bf16")),
+    "batch96": dict(seed=0x5EED0004, forms=("f16", "bf16", "fp64")),
 }
 
 
@@ -384,7 +384,7 @@ def bench_batch(args, torch, ffs):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": {"bf16": "bf16x3-split/f32-acc (f64 state)", "bf16x2": "bf16x2-split/f32-acc (f64 state)",
-                  "f16": "f16x2-split/f32-acc (f64 state)"}[form],
+                  "f16": "f16x2-split/f32-acc (f64 state)", "fp64": "f64"}[form],
         "data": "synthetic (SplitMix64 uniform[0,1) grids and weights)",
         "config": {"workload": "batch of 96 2D grids, sides 512..4096, Heat-2D/Star2D9P/Box2D9P/Box2D25P, T 16..256",
                    "grids": 96, "point_steps": pts, "form": form, "tile": info0["tile"]},

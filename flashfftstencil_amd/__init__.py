@@ -309,7 +309,7 @@ def autotune_full(dims: int, sizes: Sequence[int], weights, radius: int, steps: 
 
 
 class Batch:
-    """Independent 2D grids advanced together (one persistent launch per fused block, BF16 forms).
+    """Independent 2D grids advanced together (one persistent launch per fused block; FP64, FP16 or BF16).
 
     `problems` is a list of dicts with keys sizes, weights, radius, steps and optionally precision,
     convention, fuse (tile is always 32)."""

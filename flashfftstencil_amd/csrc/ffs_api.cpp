@@ -1028,8 +1028,9 @@ ffs_status ffs_batch_create(const ffs_problem* problems, int32_t count, ffs_batc
       return fail(FFS_ERR_UNSUPPORTED, "batch problem %d: batches run the FFT algorithm only", i);
     if (problems[i].precision != prec)
       return fail(FFS_ERR_INVALID_ARG, "batch problem %d: all problems must share one precision", i);
-    if (prec != FFS_PREC_MIXED_BF16 && prec != FFS_PREC_MIXED_BF16X2 && prec != FFS_PREC_MIXED_F16)
-      return fail(FFS_ERR_UNSUPPORTED, "batches run the BF16 and FP16 mixed forms (32-point tiles)");
+    if (prec != FFS_PREC_FP64 && prec != FFS_PREC_MIXED_BF16 && prec != FFS_PREC_MIXED_BF16X2 &&
+        prec != FFS_PREC_MIXED_F16)
+      return fail(FFS_ERR_UNSUPPORTED, "batches run the FP64, BF16 and FP16 forms (32-point tiles), not FP32");
     if (problems[i].tile != 0 && problems[i].tile != 32)
       return fail(FFS_ERR_UNSUPPORTED, "batch problem %d: tile must be 32", i);
     int64_t np = 0;

@@ -103,7 +103,7 @@ hipError_t gen_spectrum(double* H, int kind, int layout, const int32_t* axis_len
                         int32_t ntaps_per_axis, int32_t dims, int32_t convention, int32_t radius, int64_t t,
                         hipStream_t s);
 hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus);
-// One fused block of every grid of a batch (32-point BF16 family), items/prefix in device memory.
+// One fused block of every grid of a batch (32-point family: FP64, BF16, FP16), items/prefix in device memory.
 hipError_t launch_batch(const TileArgs* d_items, const int64_t* d_prefix, int32_t n, int32_t precision, hipStream_t s,
                         int num_cus);
 

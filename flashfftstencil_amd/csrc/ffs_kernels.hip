@@ -399,6 +399,40 @@ __device__ __forceinline__ void store32_f64(const TileArgs& a, const PairCoord& 
   }
 }
 
+// One tile pair of the 32-point FP64 family; the spectrum comes from LDS (single-grid launch) or from
+// the item's FP64 table in global memory (batch launch).
+template <bool H_FROM_LDS, bool OUT16>
+__device__ __forceinline__ void tile32_pair_f64(const TileArgs& a, int64_t tp, const double* __restrict__ sF,
+                                                const double2* __restrict__ sH, int lane, int q, int j) {
+  const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
+  Blk4d zr, zi, orr, oi;
+  gather32_f64(a, pc, q, j, zr, zi);
+  stage32_f64<false, 0, 0>(zr, zi, sF, lane, orr, oi);
+  stage32_f64<false, 0, 0>(orr, oi, sF, lane, zr, zi);  // Y[k0][k1], block order
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int e = (b * 4 + v) * kLanes + lane;
+      const double hr = H_FROM_LDS ? sH[e].x : a.H[e];
+      const double hi = H_FROM_LDS ? sH[e].y : a.H[1024 + e];
+      double yr = zr.b[b][v], yi = zi.b[b][v];
+      cmul(yr, yi, hr, hi);
+      zr.b[b][v] = yr;
+      zi.b[b][v] = yi;
+    }
+  if (OUT16) {
+    stage32_f64<true, 0, 1>(zr, zi, sF, lane, orr, oi);  // blocks 1 and 3
+    stage32_f64<true, 1, 1>(orr, oi, sF, lane, zr, zi);  // block 3 = the valid 16 x 16
+    store32_f64<1, 1>(a, pc, q, j, zr, zi);
+  } else {
+    stage32_f64<true, 0, 0>(zr, zi, sF, lane, orr, oi);
+    stage32_f64<true, 0, 0>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
+    store32_f64<0, 0>(a, pc, q, j, zr, zi);
+  }
+}
+
 template <bool OUT16>
 __global__ __launch_bounds__(256, 2) void k_tile32_f64(TileArgs a) {
   __shared__ double sF[kOps32F64];           // 24 KB
@@ -410,32 +444,32 @@ __global__ __launch_bounds__(256, 2) void k_tile32_f64(TileArgs a) {
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int64_t tp = gw; tp < a.pairs; tp += W) {
-    const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
-    Blk4d zr, zi, orr, oi;
-    gather32_f64(a, pc, q, j, zr, zi);
-    stage32_f64<false, 0, 0>(zr, zi, sF, lane, orr, oi);
-    stage32_f64<false, 0, 0>(orr, oi, sF, lane, zr, zi);  // Y[k0][k1], block order
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const double2 h = sH[(b * 4 + v) * kLanes + lane];
-        double yr = zr.b[b][v], yi = zi.b[b][v];
-        cmul(yr, yi, h.x, h.y);
-        zr.b[b][v] = yr;
-        zi.b[b][v] = yi;
-      }
-    if (OUT16) {
-      stage32_f64<true, 0, 1>(zr, zi, sF, lane, orr, oi);  // blocks 1 and 3
-      stage32_f64<true, 1, 1>(orr, oi, sF, lane, zr, zi);  // block 3 = the valid 16 x 16
-      store32_f64<1, 1>(a, pc, q, j, zr, zi);
-    } else {
-      stage32_f64<true, 0, 0>(zr, zi, sF, lane, orr, oi);
-      stage32_f64<true, 0, 0>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
-      store32_f64<0, 0>(a, pc, q, j, zr, zi);
+  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair_f64<true, OUT16>(a, tp, sF, sH, lane, q, j);
+}
+
+// Multi-grid launch of the FP64 form (the k_batch32_* structure): the DFT operands, the same for every
+// item, in LDS; each item's spectrum read per tile pair from its FP64 table (L2-resident).
+__global__ __launch_bounds__(256, 2) void k_batch32_f64(const TileArgs* __restrict__ items,
+                                                        const int64_t* __restrict__ prefix, int32_t n) {
+  __shared__ double sF[kOps32F64];
+  for (int i = threadIdx.x; i < kOps32F64; i += blockDim.x) sF[i] = items[0].ops[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, j = lane & 15;
+  const int64_t W = (int64_t)gridDim.x * 4;
+  const int64_t gw = block_order(items[0].linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t total = prefix[n];
+  for (int64_t idx = gw; idx < total; idx += W) {
+    int lo = 0, hi = n;  // largest g with prefix[g] <= idx
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
+    const TileArgs a = items[lo];
+    if (a.out16)
+      tile32_pair_f64<false, true>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+    else
+      tile32_pair_f64<false, false>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
   }
 }
 
@@ -1200,8 +1234,10 @@ __global__ __launch_bounds__(256, 3) void k_batch32_f16(const TileArgs* __restri
 
 hipError_t launch_batch(const TileArgs* d_items, const int64_t* d_prefix, int32_t n, int32_t precision, hipStream_t s,
                         int num_cus) {
-  const dim3 g((unsigned)num_cus * 3), b(256);
-  if (precision == PREC_BF16)
+  const dim3 g((unsigned)num_cus * (precision == 0 ? 2 : 3)), b(256);
+  if (precision == 0)
+    hipLaunchKernelGGL(k_batch32_f64, g, b, 0, s, d_items, d_prefix, n);
+  else if (precision == PREC_BF16)
     hipLaunchKernelGGL(k_batch32_bf16<3>, g, b, 0, s, d_items, d_prefix, n);
   else if (precision == PREC_BF16X2)
     hipLaunchKernelGGL(k_batch32_bf16<2>, g, b, 0, s, d_items, d_prefix, n);

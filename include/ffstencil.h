@@ -177,8 +177,8 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
 
 /* Batches of independent 2D grids (different sizes, stencils and T).  Block b of every grid that still
  * has blocks runs in ONE persistent launch over the concatenated tile-pair list of those grids; each
- * grid keeps its own fused depth, tail and spectrum.  All problems must be 2D and share one BF16 mixed
- * precision (32-point tiles).  No reference counterpart (the reference runs one grid per process);
+ * grid keeps its own fused depth, tail and spectrum.  All problems must be 2D and share one precision:
+ * FP64, FP16 or one of the BF16 forms (32-point tiles; FP32 is not batched).  No reference counterpart (the reference runs one grid per process);
  * it is the multi-grid workload's entry point.  d_in / d_out are host arrays of `count` device
  * pointers; grid i advances problems[i].steps steps. */
 typedef struct ffs_batch ffs_batch;
