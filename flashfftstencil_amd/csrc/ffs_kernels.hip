@@ -1039,6 +1039,31 @@ __device__ __forceinline__ void split8_f16(const f4& x0, const f4& x1, h8& h, h8
   l = as_h8(lw);
 }
 
+// Maximum of a non-negative v over the wave, in every lane: DPP exchanges inside each row of 16 lanes
+// (quad swaps, then the half-row and row mirrors), then the four row maxima through readlane.  For
+// non-negative floats the bit patterns order like unsigned integers, so the max is taken on those.
+// Exact (a max), and no LDS round trip as a __shfl_xor butterfly would take.
+__device__ __forceinline__ uint32_t dpp_umax(uint32_t x, int sel) {
+  int y;
+  switch (sel) {  // the DPP control must be a constant
+    case 0: y = __builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xF, 0xF, false); break;   // quad_perm [1,0,3,2]
+    case 1: y = __builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xF, 0xF, false); break;   // quad_perm [2,3,0,1]
+    case 2: y = __builtin_amdgcn_update_dpp((int)x, (int)x, 0x141, 0xF, 0xF, false); break;  // row_half_mirror
+    default: y = __builtin_amdgcn_update_dpp((int)x, (int)x, 0x140, 0xF, 0xF, false); break; // row_mirror
+  }
+  return max(x, (uint32_t)y);
+}
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+  uint32_t x = __float_as_uint(v);
+  x = dpp_umax(x, 0);
+  x = dpp_umax(x, 1);
+  x = dpp_umax(x, 2);
+  x = dpp_umax(x, 3);
+  const uint32_t r0 = __builtin_amdgcn_readlane((int)x, 0), r1 = __builtin_amdgcn_readlane((int)x, 16);
+  const uint32_t r2 = __builtin_amdgcn_readlane((int)x, 32), r3 = __builtin_amdgcn_readlane((int)x, 48);
+  return __uint_as_float(max(max(r0, r1), max(r2, r3)));
+}
+
 template <bool INV, int I0 = 0, int K0 = 0>
 __device__ __forceinline__ void stage32_f16(const Blk4& zr, const Blk4& zi, const u4* __restrict__ sF, int lane,
                                             Blk4& outr, Blk4& outi) {
@@ -1097,8 +1122,7 @@ __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, c
       zi.b[b][r] = (float)(xb[b * 4 + r] - cB);
       m = fmaxf(m, fmaxf(fabsf(zr.b[b][r]), fabsf(zi.b[b][r])));
     }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  m = wave_max_nonneg(m);
   int k;
   (void)frexpf(m, &k);  // m < 2^k
   const int e = __builtin_amdgcn_readfirstlane(min(max(a.scale_exp - k, -120), 120));
