@@ -76,6 +76,7 @@ struct ffs_plan {
   int64_t tiles[3] = {1, 1, 1};
   int64_t pairs = 0;
   int32_t h_main[3] = {0, 0, 0}, h_tail[3] = {0, 0, 0};
+  void* d_opsi = nullptr;  // 32-point family with V <= 16: inverse operands of the main and tail depths
   double* d_ops = nullptr;
   uint32_t* d_opsm = nullptr;  // mixed-form operand table
   double* d_H = nullptr;       // [main | tail], each 2*len doubles
@@ -261,7 +262,6 @@ ffs_status choose_config(ffs_plan* pl) {
   }
   const int vmin = p.dims == 1 ? 16 : 2;
   const int dd = p.dims == 1 ? 1 : p.dims;
-  const int hi_step = p.convention == FFS_CONV_REF ? 2 * r : r;
   // Maximise valid point-steps per unit time of one tile pair: 2 V^d t / max(compute, hbm), where the
   // compute cost of a pair is fixed by the tile family and precision (cycles per SIMD, measured on
   // MI355X) and the HBM cost is 16 B per valid point at ~3.26 B/cycle/SIMD (8 TB/s over 1024 SIMDs at
@@ -275,15 +275,15 @@ ffs_status choose_config(ffs_plan* pl) {
                 : p.precision == FFS_PREC_MIXED_F16 ? 3000.0
                 : p.precision == FFS_PREC_FP64     ? 31000.0  // 384 FP64 MFMAs per pair
                                                    : 9000.0;
-    // 32-point family with the valid region on window block (1, 1): reduced inverse stages, measured at
-    // 0.63 (FP16), 0.70 (BF16) and 0.74 (FP64) of a full tile pair on MI355X
+    // 32-point family with V <= 16: reduced inverse stages, measured at 0.63 (FP16), 0.70 (BF16) and
+    // 0.74 (FP64) of a full tile pair on MI355X
     const double cut = p.precision == FFS_PREC_FP64 ? 0.74 : 0.65;
     const double hbm_per_point = 2.0 * 16.0 / 3.26;
     double best = -1;
     const int64_t t0 = p.fuse > 0 ? p.fuse : 1, t1 = p.fuse > 0 ? p.fuse : std::min<int64_t>(tmax, 4096);
     for (int64_t t = t0; t <= t1; t++) {
       const double Vd = std::pow(Pm - 2.0 * r * t, dd);
-      const bool out16 = kind == ffsk::KIND_2D32 && hi_step * t == 16;
+      const bool out16 = kind == ffsk::KIND_2D32 && Pm - 2 * r * t <= 16;
       const double c = compute * (p.dims == 3 ? 16.0 : 1.0) * (out16 ? cut : 1.0);
       const double score = Vd * t / std::max(c, hbm_per_point * Vd);
       if (score > best * (1 + 1e-9)) {
@@ -340,16 +340,17 @@ void release(ffs_plan* pl) {
   if (pl->cap_stream) (void)hipStreamDestroy(pl->cap_stream);
   if (pl->d_ops) (void)hipFree(pl->d_ops);
   if (pl->d_opsm) (void)hipFree(pl->d_opsm);
+  if (pl->d_opsi) (void)hipFree(pl->d_opsi);
   if (pl->d_H) (void)hipFree(pl->d_H);
   if (pl->d_taps) (void)hipFree(pl->d_taps);
   if (pl->d_scratch) (void)hipFree(pl->d_scratch);
   delete pl;
 }
 
-// 32-point family: is the valid region of this block window block (1, 1) (rows and columns 16..31)?
-bool out16_block(const ffs_plan* pl, bool tail) {
-  const int32_t* h = tail ? pl->h_tail : pl->h_main;
-  return pl->kind == ffsk::KIND_2D32 && pl->V[0] <= 16 && pl->V[1] <= 16 && h[0] == 16 && h[1] == 16;
+// 32-point family: does the valid region fit one 16 x 16 block (reduced inverse stages)?  The main and
+// the tail block share the tiling, so both are out16 or neither; their inverse operands differ by shift.
+bool out16_block(const ffs_plan* pl) {
+  return pl->kind == ffsk::KIND_2D32 && pl->V[0] <= 16 && pl->V[1] <= 16;
 }
 
 ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, bool tail) {
@@ -384,7 +385,12 @@ ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, boo
   a.precision = p.precision;
   a.scale_exp = tail ? pl->scale_tail : pl->scale_main;
   a.linear_map = p.mapping == FFS_MAP_XCD ? 0 : 1;  // AUTO = hardware order (measured 1-3% faster)
-  a.out16 = out16_block(pl, tail) ? 1 : 0;
+  a.out16 = out16_block(pl) ? 1 : 0;
+  if (a.out16) {
+    const size_t per = pl->prob.precision == FFS_PREC_FP64 ? sizeof(double) * ffsk::kInv32F64
+                                                            : sizeof(uint32_t) * ffsk::kInv32Words;
+    a.opsi = static_cast<const char*>(pl->d_opsi) + (tail ? per : 0);
+  }
   return a;
 }
 
@@ -618,6 +624,8 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
   if (p->precision != FFS_PREC_FP64)
     chk(hipMalloc(&pl->d_opsm, sizeof(uint32_t) * std::max(ffsk::kOpsMixedLen, ffsk::kOps32Words)));
   chk(hipMalloc(&pl->d_H, sizeof(double) * 4 * pl->hlen));
+  if (out16_block(pl))
+    chk(hipMalloc(&pl->d_opsi, 2 * std::max(sizeof(double) * ffsk::kInv32F64, sizeof(uint32_t) * ffsk::kInv32Words)));
   chk(hipMalloc(&pl->d_taps, sizeof(double) * ntaps));
   if (pl->nblocks >= 2) chk(hipMalloc(&pl->d_scratch, sizeof(double) * npoints));
   if (e != hipSuccess) {
@@ -627,6 +635,18 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
   }
   chk(hipMemcpy(pl->d_taps, pl->weights.data(), sizeof(double) * ntaps, hipMemcpyHostToDevice));
   const int layout = p->precision == FFS_PREC_FP64 ? 0 : 1;
+  if (out16_block(pl)) {  // inverse operands of window columns h .. h + 15, main and tail depth
+    const int32_t hm = pl->h_main[1], ht = pl->t_tail > 0 ? pl->h_tail[1] : pl->h_main[1];
+    if (p->precision == FFS_PREC_FP64) {
+      double* o = static_cast<double*>(pl->d_opsi);
+      chk(ffsk::gen_inverse32_f64(o, hm, nullptr));
+      chk(ffsk::gen_inverse32_f64(o + ffsk::kInv32F64, ht, nullptr));
+    } else {
+      uint32_t* o = static_cast<uint32_t*>(pl->d_opsi);
+      chk(ffsk::gen_inverse32(o, p->precision == FFS_PREC_MIXED_F16, hm, nullptr));
+      chk(ffsk::gen_inverse32(o + ffsk::kInv32Words, p->precision == FFS_PREC_MIXED_F16, ht, nullptr));
+    }
+  }
   if (pl->kind == ffsk::KIND_2D32 && p->precision == FFS_PREC_FP64)
     chk(ffsk::gen_operands32_f64(pl->d_ops, nullptr));
   else if (pl->kind == ffsk::KIND_2D32)
@@ -771,9 +791,9 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   // 32-point family: output blocks over the 4 stages (4 + 4 + 4 + 4, or 4 + 4 + 2 + 1 for an out16 main
   // block) x 4 real GEMMs x (6, 4 or 3) K=32 MFMAs
   if (pl->kind == ffsk::KIND_2D32 && p.precision == FFS_PREC_FP64)  // x 3 Gauss GEMMs x 8 K=4 chunks
-    per_pair = (out16_block(pl, false) ? 11.0 : 16.0) * 3.0 * 8.0 * 2.0 * 16 * 16 * 4;
+    per_pair = (out16_block(pl) ? 11.0 : 16.0) * 3.0 * 8.0 * 2.0 * 16 * 16 * 4;
   else if (pl->kind == ffsk::KIND_2D32)
-    per_pair = (out16_block(pl, false) ? 11.0 : 16.0) * 4.0 *
+    per_pair = (out16_block(pl) ? 11.0 : 16.0) * 4.0 *
                (p.precision == FFS_PREC_MIXED_BF16 ? 6.0 : (p.precision == FFS_PREC_MIXED_F16 ? 3.0 : 4.0)) *
                2.0 * 16 * 16 * 32;
   info->flops_per_block = per_pair * (double)pl->pairs;

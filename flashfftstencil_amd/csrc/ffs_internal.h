@@ -60,10 +60,12 @@ struct TileArgs {
   int32_t scale_exp;  // FP16 form: the tile pair is scaled so that its largest fluctuation is below 2^scale_exp
   double dc_gain;  // K(0)^t: gain of the constant mode (mixed forms add mean * gain back in FP64)
   int32_t linear_map;  // 1: workgroups take tiles in hardware order; 0: XCD-contiguous bands
-  // 32-point mixed forms: 1 when the valid region is exactly window block (1, 1), i.e. h0 = h1 = 16 and
-  // V0, V1 <= 16.  The two inverse stages then compute only the outputs of that block (2 and 1 of 4
-  // output blocks); every value they keep is produced by the same instructions as in the full stages.
+  // 32-point family: 1 when the valid region fits one 16 x 16 block (V0, V1 <= 16).  The two inverse
+  // stages then compute only output block 0 (2 and 1 of 4 output blocks) with the inverse operands of
+  // window columns h .. h + 15 (`opsi`), so output index m is window position h + m: every value kept is
+  // the one the full stages would produce at that position, from the same operands.
   int32_t out16;
+  const void* opsi;  // out16: inverse operands (kInv32Words words or kInv32F64 doubles, see below)
 };
 
 // Direct (non-FFT) FP64 kernels with temporal blocking (ffs_direct.hip).
@@ -96,6 +98,13 @@ hipError_t launch_direct(const DirectArgs& a, int dims, int radius, bool star, h
 hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hipStream_t s);
 hipError_t gen_operands32(uint32_t* ops32, bool f16, hipStream_t s);
 hipError_t gen_operands32_f64(double* ops, hipStream_t s);
+// Inverse operands of an out16 block, output column j <- window column (j + shift) mod 32:
+//   mixed forms: u4 per lane, [arr: R, I, NI][term 0..2][lane] (the kOps32Words entries of block Kb = 0);
+//   FP64: [R][chunk v][arr: R, S, D][lane] doubles (the kOps32F64 entries of block Kb = 0).
+constexpr int kInv32Words = 3 * 3 * 64 * 4;   // 2304 words
+constexpr int kInv32F64 = 2 * 4 * 3 * 64;     // 1536 doubles
+hipError_t gen_inverse32(uint32_t* out, bool f16, int32_t shift, hipStream_t s);
+hipError_t gen_inverse32_f64(double* out, int32_t shift, hipStream_t s);
 // d[i] = output i of SplitMix64(seed) as a double in [0, 1) (the host ffs_fill_uniform sequence)
 hipError_t fill_uniform_device(double* d, int64_t n, uint64_t seed, hipStream_t s);
 // layout 0: FP64 accumulator order (row freq q + 4v); layout 1: FP32 accumulator order (row freq 4q + v)

@@ -299,31 +299,35 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
 // so register v of block (R, I) is the A operand of contraction rows 16R + 4v .. + 3 (lane (i + 16k)
 // holds M[16R + 4v + k][16I + i]) and a stage Out = Z^T F is, per output block (I, Kb), 8 chunks x 3
 // Gauss GEMMs = 24 v_mfma_f64_16x16x4f64 into three accumulators, as in the 16-point kernel.  With OUT16
-// (valid region = window block (1, 1)) the inverse stages compute only the outputs kept: blocks (I, 1),
-// then (1, 1).  DFT operands (24 KB) and the FP64 spectrum (16 KB) are staged in LDS per workgroup.
+// (V <= 16) the inverse stages compute only output block 0 from the inverse operands of window columns
+// h .. h + 15: blocks (I, 0), then (0, 0).  DFT operands (24 KB), the FP64 spectrum (16 KB) and the
+// out16 inverse operands (12 KB) are staged in LDS per workgroup.
 // ---------------------------------------------------------------------------------------------
 struct Blk4d {
   d4 b[4];
 };
 
-template <bool INV, int I0, int K0>
+// Output blocks (I, Kb), I < NI, Kb < NK, from a table holding KBS column blocks per chunk: the full
+// operands (NI = NK = KBS = 2), or for the inverse stages of an out16 tile the shifted one-block table
+// (KBS = 1): NI = 2, NK = 1, then NI = NK = 1.
+template <bool INV, int NI, int NK, int KBS>
 __device__ __forceinline__ void stage32_f64(const Blk4d& zr, const Blk4d& zi, const double* __restrict__ sF, int lane,
                                             Blk4d& outr, Blk4d& outi) {
 #pragma unroll
-  for (int I = I0; I < 2; I++) {
+  for (int I = 0; I < NI; I++) {
     double sum[2][4];
 #pragma unroll
     for (int R = 0; R < 2; R++)
 #pragma unroll
       for (int v = 0; v < 4; v++) sum[R][v] = zr.b[2 * R + I][v] + zi.b[2 * R + I][v];
 #pragma unroll
-    for (int Kb = K0; Kb < 2; Kb++) {
+    for (int Kb = 0; Kb < NK; Kb++) {
       d4 k1 = {0.0, 0.0, 0.0, 0.0}, k2 = k1, k3 = k1;
 #pragma unroll
       for (int R = 0; R < 2; R++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const double* f = sF + (((R * 4 + v) * 2 + Kb) * 3) * kLanes + lane;
+          const double* f = sF + (((R * 4 + v) * KBS + Kb) * 3) * kLanes + lane;
           const double fr = f[0], fs = f[kLanes], fd = f[2 * kLanes];
           k1 = FFS_MFMA64(sum[R][v], fr, k1);
           k2 = FFS_MFMA64(zr.b[2 * R + I][v], INV ? fs : fd, k2);
@@ -373,19 +377,18 @@ __device__ __forceinline__ void gather32_f64(const TileArgs& a, const PairCoord&
   }
 }
 
-// Store the valid outputs of blocks (R, C) with R >= R0, C >= C0 (OUT16: only (1, 1)).
-template <int R0, int C0>
+// Store the valid outputs of all four blocks (window rows 16R + q + 4v, columns 16C + j).
 __device__ __forceinline__ void store32_f64(const TileArgs& a, const PairCoord& pc, int q, int j, const Blk4d& yr,
                                             const Blk4d& yi) {
 #pragma unroll
-  for (int C = C0; C < 2; C++) {
+  for (int C = 0; C < 2; C++) {
     const int wc = 16 * C + j;
     const int64_t cA = pc.orgA + wc - a.h1, cB = pc.orgB + wc - a.h1;
     const bool colok = wc >= a.h1 && wc < a.h1 + a.V1;
     const bool okA = colok && cA < a.n1;
     const bool okB = colok && pc.hasB && cB < a.n1;
 #pragma unroll
-    for (int R = R0; R < 2; R++)
+    for (int R = 0; R < 2; R++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int row = 16 * R + q + 4 * v;
@@ -399,16 +402,36 @@ __device__ __forceinline__ void store32_f64(const TileArgs& a, const PairCoord& 
   }
 }
 
+// Store of an out16 tile pair: output block 0, lane (j + 16q), reg v = valid point (q + 4v, j) (window
+// position (h0 + q + 4v, h1 + j)).
+__device__ __forceinline__ void store16_f64(const TileArgs& a, const PairCoord& pc, int q, int j, const d4& yr,
+                                            const d4& yi) {
+  const int64_t cA = pc.orgA + j, cB = pc.orgB + j;
+  const bool okA = j < a.V1 && cA < a.n1;
+  const bool okB = j < a.V1 && pc.hasB && cB < a.n1;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int row = q + 4 * v;
+    const int64_t y = pc.r0 + row;
+    if (row < a.V0 && y < a.n0) {
+      double* drow = a.dst + y * a.n1;
+      if (okA) drow[cA] = yr[v];
+      if (okB) drow[cB] = yi[v];
+    }
+  }
+}
+
 // One tile pair of the 32-point FP64 family; the spectrum comes from LDS (single-grid launch) or from
 // the item's FP64 table in global memory (batch launch).
 template <bool H_FROM_LDS, bool OUT16>
 __device__ __forceinline__ void tile32_pair_f64(const TileArgs& a, int64_t tp, const double* __restrict__ sF,
-                                                const double2* __restrict__ sH, int lane, int q, int j) {
+                                                const double2* __restrict__ sH, const double* __restrict__ sG,
+                                                int lane, int q, int j) {
   const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
   Blk4d zr, zi, orr, oi;
   gather32_f64(a, pc, q, j, zr, zi);
-  stage32_f64<false, 0, 0>(zr, zi, sF, lane, orr, oi);
-  stage32_f64<false, 0, 0>(orr, oi, sF, lane, zr, zi);  // Y[k0][k1], block order
+  stage32_f64<false, 2, 2, 2>(zr, zi, sF, lane, orr, oi);
+  stage32_f64<false, 2, 2, 2>(orr, oi, sF, lane, zr, zi);  // Y[k0][k1], block order
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int b = 0; b < 4; b++)
@@ -423,13 +446,13 @@ __device__ __forceinline__ void tile32_pair_f64(const TileArgs& a, int64_t tp, c
       zi.b[b][v] = yi;
     }
   if (OUT16) {
-    stage32_f64<true, 0, 1>(zr, zi, sF, lane, orr, oi);  // blocks 1 and 3
-    stage32_f64<true, 1, 1>(orr, oi, sF, lane, zr, zi);  // block 3 = the valid 16 x 16
-    store32_f64<1, 1>(a, pc, q, j, zr, zi);
+    stage32_f64<true, 2, 1, 1>(zr, zi, sG, lane, orr, oi);  // blocks 0 and 2: window rows h0 ..
+    stage32_f64<true, 1, 1, 1>(orr, oi, sG, lane, zr, zi);  // block 0 = the valid region
+    store16_f64(a, pc, q, j, zr.b[0], zi.b[0]);
   } else {
-    stage32_f64<true, 0, 0>(zr, zi, sF, lane, orr, oi);
-    stage32_f64<true, 0, 0>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
-    store32_f64<0, 0>(a, pc, q, j, zr, zi);
+    stage32_f64<true, 2, 2, 2>(zr, zi, sF, lane, orr, oi);
+    stage32_f64<true, 2, 2, 2>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
+    store32_f64(a, pc, q, j, zr, zi);
   }
 }
 
@@ -437,14 +460,17 @@ template <bool OUT16>
 __global__ __launch_bounds__(256, 2) void k_tile32_f64(TileArgs a) {
   __shared__ double sF[kOps32F64];           // 24 KB
   __shared__ double2 sH[4 * 4 * kLanes];     // spectrum [block][reg][lane] (re, im), 16 KB
+  __shared__ double sG[OUT16 ? kInv32F64 : 1];  // out16 inverse operands, 12 KB
   for (int i = threadIdx.x; i < kOps32F64; i += blockDim.x) sF[i] = a.ops[i];
+  if (OUT16)
+    for (int i = threadIdx.x; i < kInv32F64; i += blockDim.x) sG[i] = static_cast<const double*>(a.opsi)[i];
   for (int i = threadIdx.x; i < 1024; i += blockDim.x) sH[i] = double2{a.H[i], a.H[1024 + i]};
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair_f64<true, OUT16>(a, tp, sF, sH, lane, q, j);
+  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair_f64<true, OUT16>(a, tp, sF, sH, sG, lane, q, j);
 }
 
 // Multi-grid launch of the FP64 form (the k_batch32_* structure): the DFT operands, the same for every
@@ -467,9 +493,9 @@ __global__ __launch_bounds__(256, 2) void k_batch32_f64(const TileArgs* __restri
     }
     const TileArgs a = items[lo];
     if (a.out16)
-      tile32_pair_f64<false, true>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+      tile32_pair_f64<false, true>(a, idx - prefix[lo], sF, nullptr, static_cast<const double*>(a.opsi), lane, q, j);
     else
-      tile32_pair_f64<false, false>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+      tile32_pair_f64<false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
   }
 }
 
@@ -833,29 +859,32 @@ __device__ __forceinline__ f4 prod32_small(const bf8& ah, const bf8& am, const b
   return acc;
 }
 
+// KBS: column blocks per arr in the table (2: the full operands; 1: the out16 inverse operands).
+template <int KBS>
 __device__ __forceinline__ void load_f32ops(const u4* __restrict__ sF, int arr, int Kb, int lane, bf8 (&F)[3]) {
 #pragma unroll
-  for (int t = 0; t < 3; t++) F[t] = as_bf8(sF[((arr * 2 + Kb) * 3 + t) * kLanes + lane]);
+  for (int t = 0; t < 3; t++) F[t] = as_bf8(sF[((arr * KBS + Kb) * 3 + t) * kLanes + lane]);
 }
 
-// Output blocks (I, Kb) with I in [I0, 2) and Kb in [K0, 2): all four for the forward stages; for the
-// inverse stages of an out16 tile Kb = 1 only (the valid window rows 16..31), then (1, 1) only.
-template <bool INV, int NTERMS, int I0 = 0, int K0 = 0>
+// Output blocks (I, Kb), I < NI, Kb < NK, from a table of KBS column blocks: all four from the full
+// operands (NI = NK = KBS = 2); for the inverse stages of an out16 tile block Kb = 0 of the shifted
+// one-block table (the valid window rows), NI = 2, NK = KBS = 1, then NI = NK = KBS = 1.
+template <bool INV, int NTERMS, int NI = 2, int NK = 2, int KBS = 2>
 __device__ __forceinline__ void stage32(const Blk4& zr, const Blk4& zi, const u4* __restrict__ sF, int lane, Blk4& outr,
                                         Blk4& outi) {
 #pragma unroll
-  for (int I = I0; I < 2; I++) {
+  for (int I = 0; I < NI; I++) {
     bf8 rh, rm, rl, ih, im, il;
     f4 x0 = zr.b[I], x1 = zr.b[2 + I], y0 = zi.b[I], y1 = zi.b[2 + I];
     mfma_settle(x0, x1, y0, y1);
     split8<NTERMS>(x0, x1, rh, rm, rl);
     split8<NTERMS>(y0, y1, ih, im, il);
 #pragma unroll
-    for (int Kb = K0; Kb < 2; Kb++) {
+    for (int Kb = 0; Kb < NK; Kb++) {
       bf8 Fr[3], Fi[3], Fn[3];
-      load_f32ops(sF, 0, Kb, lane, Fr);
-      load_f32ops(sF, 1, Kb, lane, Fi);
-      load_f32ops(sF, 2, Kb, lane, Fn);
+      load_f32ops<KBS>(sF, 0, Kb, lane, Fr);
+      load_f32ops<KBS>(sF, 1, Kb, lane, Fi);
+      load_f32ops<KBS>(sF, 2, Kb, lane, Fn);
       f4 re = {0.f, 0.f, 0.f, 0.f}, iv = re;
       // forward: Re = zr Fr + zi (-Fi), Im = zr Fi + zi Fr;  inverse: Re = zr Fr + zi Fi, Im = zr (-Fi) + zi Fr
       // small split products first, the two dominant ones last (two full-magnitude roundings per output)
@@ -930,15 +959,16 @@ __device__ __forceinline__ void store32(const TileArgs& a, const PairCoord& pc, 
   }
 }
 
-// Valid region of an out16 tile pair: output block 3 = window rows and columns 16..31.
+// Valid region of an out16 tile pair: output block 0, lane (j + 16q), reg v = valid point (4q + v, j)
+// (window position (h0 + 4q + v, h1 + j)).
 __device__ __forceinline__ void store16(const TileArgs& a, const PairCoord& pc, int q, int j, const f4& yr,
                                         const f4& yi, double addA, double addB, double scale = 1.0) {
-  const int64_t cA = pc.orgA + j, cB = pc.orgB + j;  // window column 16 + j
+  const int64_t cA = pc.orgA + j, cB = pc.orgB + j;  // window column h1 + j
   const bool okA = j < a.V1 && cA < a.n1;
   const bool okB = j < a.V1 && pc.hasB && cB < a.n1;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
-    const int row = 4 * q + v;  // window row 16 + row
+    const int row = 4 * q + v;  // window row h0 + row
     const int64_t y = pc.r0 + row;
     if (row < a.V0 && y < a.n0) {
       double* drow = a.dst + y * a.n1;
@@ -952,7 +982,8 @@ __device__ __forceinline__ void store16(const TileArgs& a, const PairCoord& pc, 
 // grid's FP64 table in global memory (batch launch, where every grid has its own stencil and depth).
 template <int NTERMS, bool H_FROM_LDS, bool OUT16>
 __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const u4* __restrict__ sF,
-                                            const f2* __restrict__ sH, int lane, int q, int j) {
+                                            const f2* __restrict__ sH, const u4* __restrict__ sG, int lane, int q,
+                                            int j) {
   const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
   double xa[16], xb[16];
   gather32(a, pc, q, j, xa, xb);
@@ -984,9 +1015,9 @@ __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const
       zi.b[b][v] = yi;
     }
   if (OUT16) {
-    stage32<true, NTERMS, 0, 1>(zr, zi, sF, lane, orr, oi);  // blocks 1 and 3
-    stage32<true, NTERMS, 1, 1>(orr, oi, sF, lane, zr, zi);  // block 3 = the valid 16 x 16
-    store16(a, pc, q, j, zr.b[3], zi.b[3], cA * a.dc_gain, cB * a.dc_gain);
+    stage32<true, NTERMS, 2, 1, 1>(zr, zi, sG, lane, orr, oi);  // blocks 0 and 2: window rows h0 ..
+    stage32<true, NTERMS, 1, 1, 1>(orr, oi, sG, lane, zr, zi);  // block 0 = the valid region
+    store16(a, pc, q, j, zr.b[0], zi.b[0], cA * a.dc_gain, cB * a.dc_gain);
   } else {
     stage32<true, NTERMS>(zr, zi, sF, lane, orr, oi);
     stage32<true, NTERMS>(orr, oi, sF, lane, zr, zi);  // X'[n0][n1], block order
@@ -1064,24 +1095,24 @@ __device__ __forceinline__ float wave_max_nonneg(float v) {
   return __uint_as_float(max(max(r0, r1), max(r2, r3)));
 }
 
-template <bool INV, int I0 = 0, int K0 = 0>
+template <bool INV, int NI = 2, int NK = 2, int KBS = 2>
 __device__ __forceinline__ void stage32_f16(const Blk4& zr, const Blk4& zi, const u4* __restrict__ sF, int lane,
                                             Blk4& outr, Blk4& outi) {
 #pragma unroll
-  for (int I = I0; I < 2; I++) {
+  for (int I = 0; I < NI; I++) {
     h8 rh, rl, ih, il;
     f4 x0 = zr.b[I], x1 = zr.b[2 + I], y0 = zi.b[I], y1 = zi.b[2 + I];
     mfma_settle(x0, x1, y0, y1);
     split8_f16(x0, x1, rh, rl);
     split8_f16(y0, y1, ih, il);
 #pragma unroll
-    for (int Kb = K0; Kb < 2; Kb++) {
+    for (int Kb = 0; Kb < NK; Kb++) {
       h8 Fr[2], Fi[2], Fn[2];
 #pragma unroll
       for (int t = 0; t < 2; t++) {
-        Fr[t] = as_h8(sF[((0 * 2 + Kb) * 3 + t) * kLanes + lane]);
-        Fi[t] = as_h8(sF[((1 * 2 + Kb) * 3 + t) * kLanes + lane]);
-        Fn[t] = as_h8(sF[((2 * 2 + Kb) * 3 + t) * kLanes + lane]);
+        Fr[t] = as_h8(sF[((0 * KBS + Kb) * 3 + t) * kLanes + lane]);
+        Fi[t] = as_h8(sF[((1 * KBS + Kb) * 3 + t) * kLanes + lane]);
+        Fn[t] = as_h8(sF[((2 * KBS + Kb) * 3 + t) * kLanes + lane]);
       }
       // forward: Re = zr Fr + zi (-Fi), Im = zr Fi + zi Fr;  inverse: Re = zr Fr + zi Fi, Im = zr (-Fi) + zi Fr
       const h8* Gi = INV ? Fn : Fi;  // multiplies zr into Im
@@ -1107,7 +1138,8 @@ __device__ __forceinline__ void stage32_f16(const Blk4& zr, const Blk4& zi, cons
 
 template <bool H_FROM_LDS, bool OUT16>
 __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, const u4* __restrict__ sF,
-                                                const f2* __restrict__ sH, int lane, int q, int j) {
+                                                const f2* __restrict__ sH, const u4* __restrict__ sG, int lane,
+                                                int q, int j) {
   const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
   double xa[16], xb[16];
   gather32(a, pc, q, j, xa, xb);
@@ -1150,9 +1182,9 @@ __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, c
       zi.b[b][v] = yi;
     }
   if (OUT16) {
-    stage32_f16<true, 0, 1>(zr, zi, sF, lane, orr, oi);
-    stage32_f16<true, 1, 1>(orr, oi, sF, lane, zr, zi);
-    store16(a, pc, q, j, zr.b[3], zi.b[3], cA * a.dc_gain, cB * a.dc_gain, ldexp(1.0, -e));
+    stage32_f16<true, 2, 1, 1>(zr, zi, sG, lane, orr, oi);
+    stage32_f16<true, 1, 1, 1>(orr, oi, sG, lane, zr, zi);
+    store16(a, pc, q, j, zr.b[0], zi.b[0], cA * a.dc_gain, cB * a.dc_gain, ldexp(1.0, -e));
   } else {
     stage32_f16<true>(zr, zi, sF, lane, orr, oi);
     stage32_f16<true>(orr, oi, sF, lane, zr, zi);
@@ -1170,18 +1202,27 @@ __device__ __forceinline__ void stage_ops32(u4* sF, const uint32_t* __restrict__
   }
 }
 
+// Inverse operands of an out16 block (9 KB), already in u4 order.
+template <bool OUT16>
+__device__ __forceinline__ void stage_inv32(u4* sG, const void* opsi) {
+  if (OUT16)
+    for (int i = threadIdx.x; i < 3 * 3 * kLanes; i += blockDim.x) sG[i] = static_cast<const u4*>(opsi)[i];
+}
+
 template <int NTERMS, bool OUT16>
 __global__ __launch_bounds__(256, 3) void k_tile32_bf16(TileArgs a) {
   __shared__ u4 sF[3 * 2 * 3 * kLanes];  // B operands, 18 KB
   __shared__ f2 sH[4 * 4 * kLanes];      // spectrum [block][reg][lane] (re, im), 8 KB
+  __shared__ u4 sG[OUT16 ? 3 * 3 * kLanes : 1];  // out16 inverse operands, 9 KB
   stage_ops32(sF, a.opsm);
+  stage_inv32<OUT16>(sG, a.opsi);
   for (int i = threadIdx.x; i < 1024; i += blockDim.x) sH[i] = f2{(float)a.H[i], (float)a.H[1024 + i]};
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair<NTERMS, true, OUT16>(a, tp, sF, sH, lane, q, j);
+  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair<NTERMS, true, OUT16>(a, tp, sF, sH, sG, lane, q, j);
 }
 
 // Multi-grid launch: one fused block of every grid still running, as one persistent launch over the
@@ -1210,9 +1251,9 @@ __global__ __launch_bounds__(256, 3) void k_batch32_bf16(const TileArgs* __restr
     }
     const TileArgs a = items[lo];
     if (a.out16)
-      tile32_pair<NTERMS, false, true>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+      tile32_pair<NTERMS, false, true>(a, idx - prefix[lo], sF, nullptr, static_cast<const u4*>(a.opsi), lane, q, j);
     else
-      tile32_pair<NTERMS, false, false>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+      tile32_pair<NTERMS, false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
   }
 }
 
@@ -1222,14 +1263,16 @@ template <bool OUT16>
 __global__ __launch_bounds__(256, OUT16 ? 4 : 3) void k_tile32_f16(TileArgs a) {
   __shared__ u4 sF[3 * 2 * 3 * kLanes];
   __shared__ f2 sH[4 * 4 * kLanes];
+  __shared__ u4 sG[OUT16 ? 3 * 3 * kLanes : 1];
   stage_ops32(sF, a.opsm);
+  stage_inv32<OUT16>(sG, a.opsi);
   for (int i = threadIdx.x; i < 1024; i += blockDim.x) sH[i] = f2{(float)a.H[i], (float)a.H[1024 + i]};
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair_f16<true, OUT16>(a, tp, sF, sH, lane, q, j);
+  for (int64_t tp = gw; tp < a.pairs; tp += W) tile32_pair_f16<true, OUT16>(a, tp, sF, sH, sG, lane, q, j);
 }
 
 __global__ __launch_bounds__(256, 3) void k_batch32_f16(const TileArgs* __restrict__ items,
@@ -1250,9 +1293,9 @@ __global__ __launch_bounds__(256, 3) void k_batch32_f16(const TileArgs* __restri
     }
     const TileArgs a = items[lo];
     if (a.out16)
-      tile32_pair_f16<false, true>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+      tile32_pair_f16<false, true>(a, idx - prefix[lo], sF, nullptr, static_cast<const u4*>(a.opsi), lane, q, j);
     else
-      tile32_pair_f16<false, false>(a, idx - prefix[lo], sF, nullptr, lane, q, j);
+      tile32_pair_f16<false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
   }
 }
 
@@ -1598,6 +1641,45 @@ __global__ void k_gen_operands32(uint32_t* t, int32_t f16) {
   }
 }
 
+// Inverse operands of an out16 block (layouts in ffs_internal.h): the Kb = 0 entries of the operand
+// tables with output column j taken from window column (j + shift) mod 32.
+__global__ void k_gen_inverse32(uint32_t* out, int32_t f16, int32_t shift) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 3 * 3 * kLanes) return;
+  const int lane = idx % kLanes, term = (idx / kLanes) % 3, arr = idx / (3 * kLanes);
+  const int q = lane >> 4, j = lane & 15;
+  for (int w = 0; w < 4; w++) {
+    uint32_t word = 0;
+    for (int half = 0; half < 2; half++) {
+      const int e = 2 * w + half;
+      double fr, fi;
+      dft_entry(16 * (e >> 2) + 4 * q + (e & 3), (j + shift) & 31, 32, fr, fi);
+      const double v = arr == 0 ? fr : (arr == 1 ? fi : -fi);
+      uint16_t bits;
+      if (f16) {
+        const _Float16 h = (_Float16)(float)v;
+        const _Float16 l = (_Float16)(float)(v - (double)h);
+        bits = __builtin_bit_cast(uint16_t, term == 0 ? h : (term == 1 ? l : (_Float16)0.0f));
+      } else {
+        __bf16 h, m, l;
+        split3_bf16(v, h, m, l);
+        bits = __builtin_bit_cast(uint16_t, term == 0 ? h : (term == 1 ? m : l));
+      }
+      word |= (uint32_t)bits << (16 * half);
+    }
+    out[idx * 4 + w] = word;
+  }
+}
+
+__global__ void k_gen_inverse32_f64(double* out, int32_t shift) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= kInv32F64) return;
+  const int lane = idx % kLanes, arr = (idx / kLanes) % 3, v = (idx / (3 * kLanes)) % 4, R = idx / (12 * kLanes);
+  double fr, fi;
+  dft_entry(16 * R + 4 * v + (lane >> 4), ((lane & 15) + shift) & 31, 32, fr, fi);
+  out[idx] = arr == 0 ? fr : (arr == 1 ? fr + fi : fi - fr);
+}
+
 // 32-point FP64 B operands (layout kOps32F64): one thread per entry.
 __global__ void k_gen_operands32_f64(double* t) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1722,6 +1804,16 @@ __global__ void k_fill_uniform(double* d, int64_t n, uint64_t seed) {
 
 hipError_t fill_uniform_device(double* d, int64_t n, uint64_t seed, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_uniform, dim3(1024), dim3(256), 0, s, d, n, seed);
+  return hipGetLastError();
+}
+
+hipError_t gen_inverse32(uint32_t* out, bool f16, int32_t shift, hipStream_t s) {
+  hipLaunchKernelGGL(k_gen_inverse32, dim3((3 * 3 * kLanes + 255) / 256), dim3(256), 0, s, out, (int32_t)f16, shift);
+  return hipGetLastError();
+}
+
+hipError_t gen_inverse32_f64(double* out, int32_t shift, hipStream_t s) {
+  hipLaunchKernelGGL(k_gen_inverse32_f64, dim3((kInv32F64 + 255) / 256), dim3(256), 0, s, out, shift);
   return hipGetLastError();
 }
 

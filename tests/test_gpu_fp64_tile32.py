@@ -1,8 +1,8 @@
 """GPU parity of the FP64 form on 32-point tiles (k_tile32_f64) against the FP64 direct-sweep oracle.
 
 Bar: max|gpu - oracle| / max|oracle| <= 1e-12, as for the 16-point FP64 form.  The depths cover the
-reduced inverse stages (valid region on window block (1, 1): reference convention with 2 r t = 16) and
-the full ones, with tail blocks of another depth, both index conventions, and grids smaller than a tile.
+reduced inverse stages (valid width V = 32 - 2 r t <= 16, at the window offset of either index
+convention) and the full ones, with tail blocks of another depth, and grids smaller than a tile.
 """
 import numpy as np
 import pytest
