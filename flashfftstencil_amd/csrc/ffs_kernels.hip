@@ -344,21 +344,36 @@ __device__ __forceinline__ void stage32_f64(const Blk4d& zr, const Blk4d& zi, co
   }
 }
 
+// Interior windows are read through a raw buffer view based at the window's first element: one 32-bit
+// per-lane offset plus a wave-uniform offset per row, so no 64-bit address arithmetic per load. The flags
+// word is the gfx9 raw-buffer setting (32-bit data format). A view with zero records reads as zeros, which
+// stands in for the absent second tile of an odd pair. Offsets stay below 2^31 for n1 <= 2^22.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t window_rsrc(const double* base, bool live) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), 0, live ? 0x7fffffff : 0, 0x00020000);
+}
+
+__device__ __forceinline__ double window_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
 // Window values of the pair in block order: x.b[2R + C][v] = window[16R + q + 4v][16C + j].
 __device__ __forceinline__ void gather32_f64(const TileArgs& a, const PairCoord& pc, int q, int j, Blk4d& xa,
                                              Blk4d& xb) {
   const double* __restrict__ src = a.src;
   const int64_t rw = pc.r0 - a.h0, cwA = pc.orgA - a.h1, cwB = pc.orgB - a.h1;
-  const bool interior = rw >= 0 && rw + 32 <= a.n0 && cwA >= 0 && (pc.hasB ? cwB : cwA) + 32 <= a.n1;
+  const bool interior = rw >= 0 && rw + 32 <= a.n0 && cwA >= 0 && (pc.hasB ? cwB : cwA) + 32 <= a.n1 &&
+                        a.n1 <= (1 << 22);
   if (interior) {
-    const double* p = src + (rw + q) * a.n1 + cwA + j;
+    const double* baseA = src + rw * a.n1 + cwA;
+    const __amdgpu_buffer_rsrc_t ra = window_rsrc(baseA, true), rb = window_rsrc(baseA + a.V1, pc.hasB);
+    const uint32_t voff = ((uint32_t)q * (uint32_t)a.n1 + (uint32_t)j) * 8u;
 #pragma unroll
     for (int b = 0; b < 4; b++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int64_t off = (int64_t)(16 * (b >> 1) + 4 * v) * a.n1 + 16 * (b & 1);
-        xa.b[b][v] = p[off];
-        xb.b[b][v] = pc.hasB ? p[off + a.V1] : 0.0;
+        const uint32_t soff = ((uint32_t)(16 * (b >> 1) + 4 * v) * (uint32_t)a.n1 + 16 * (b & 1)) * 8u;
+        xa.b[b][v] = window_load(ra, voff, soff);
+        xb.b[b][v] = window_load(rb, voff, soff);
       }
   } else {
 #pragma unroll
@@ -907,16 +922,19 @@ __device__ __forceinline__ void gather32(const TileArgs& a, const PairCoord& pc,
                                          double (&xb)[16]) {
   const double* __restrict__ src = a.src;
   const int64_t rw = pc.r0 - a.h0, cwA = pc.orgA - a.h1, cwB = pc.orgB - a.h1;
-  const bool interior = rw >= 0 && rw + 32 <= a.n0 && cwA >= 0 && (pc.hasB ? cwB : cwA) + 32 <= a.n1;
+  const bool interior = rw >= 0 && rw + 32 <= a.n0 && cwA >= 0 && (pc.hasB ? cwB : cwA) + 32 <= a.n1 &&
+                        a.n1 <= (1 << 22);
   if (interior) {
-    const double* p = src + (rw + 4 * q) * a.n1 + cwA + j;
+    const double* baseA = src + rw * a.n1 + cwA;
+    const __amdgpu_buffer_rsrc_t ra = window_rsrc(baseA, true), rb = window_rsrc(baseA + a.V1, pc.hasB);
+    const uint32_t voff = ((uint32_t)(4 * q) * (uint32_t)a.n1 + (uint32_t)j) * 8u;
 #pragma unroll
     for (int b = 0; b < 4; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int64_t off = (int64_t)(16 * (b >> 1) + r) * a.n1 + 16 * (b & 1);
-        xa[b * 4 + r] = p[off];
-        xb[b * 4 + r] = pc.hasB ? p[off + a.V1] : 0.0;
+        const uint32_t soff = ((uint32_t)(16 * (b >> 1) + r) * (uint32_t)a.n1 + 16 * (b & 1)) * 8u;
+        xa[b * 4 + r] = window_load(ra, voff, soff);
+        xb[b * 4 + r] = window_load(rb, voff, soff);
       }
   } else {
 #pragma unroll
