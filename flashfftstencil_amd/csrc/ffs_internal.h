@@ -42,7 +42,8 @@ inline int64_t spectrum_len(int kind) {  // complex values per spectrum (one dep
 constexpr int kOps32Words = 3 * 2 * 3 * 4 * kLanes;  // 4608
 
 // 32-point FP64 B operands (KIND_2D32, FP64 form): [R][chunk v][column block Kb][arr: R, S, D][lane], lane
-// (n + 16k) holding entry (row 16 R + 4 v + k, column 16 Kb + n) of Fr, Fr + Fi or Fi - Fr.
+// (n + 16k) holding entry (row 16 R + 4 v + k, column 16 Kb + n) of Fr, -(Fr + Fi) or Fi - Fr (the sum is
+// stored negated so that both output accumulators chain from the shared Gauss product).
 constexpr int kOps32F64 = 2 * 4 * 2 * 3 * kLanes;  // 3072 doubles
 
 struct TileArgs {
@@ -100,7 +101,7 @@ hipError_t gen_operands32(uint32_t* ops32, bool f16, hipStream_t s);
 hipError_t gen_operands32_f64(double* ops, hipStream_t s);
 // Inverse operands of an out16 block, output column j <- window column (j + shift) mod 32:
 //   mixed forms: u4 per lane, [arr: R, I, NI][term 0..2][lane] (the kOps32Words entries of block Kb = 0);
-//   FP64: [R][chunk v][arr: R, S, D][lane] doubles (the kOps32F64 entries of block Kb = 0).
+//   FP64: [R][chunk v][arr: Fr, -(Fr + Fi), Fi - Fr][lane] doubles (the kOps32F64 entries of block Kb = 0).
 constexpr int kInv32Words = 3 * 3 * 64 * 4;   // 2304 words
 constexpr int kInv32F64 = 2 * 4 * 3 * 64;     // 1536 doubles
 hipError_t gen_inverse32(uint32_t* out, bool f16, int32_t shift, hipStream_t s);

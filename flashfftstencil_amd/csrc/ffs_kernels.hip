@@ -309,7 +309,10 @@ struct Blk4d {
 
 // Output blocks (I, Kb), I < NI, Kb < NK, from a table holding KBS column blocks per chunk: the full
 // operands (NI = NK = KBS = 2), or for the inverse stages of an out16 tile the shifted one-block table
-// (KBS = 1): NI = 2, NK = 1, then NI = NK = 1.
+// (KBS = 1): NI = 2, NK = 1, then NI = NK = 1.  Gauss form with the tables r = Fr, n = -(Fr + Fi),
+// d = Fi - Fr: k1 = (zr + zi) r is accumulated first, and both outputs chain from it (the MFMA's C input
+// is k1, its result a new accumulator), so no output additions are needed:
+//   forward  Re = k1 + zi n, Im = k1 + zr d;   inverse  Re = k1 + zi d, Im = k1 + zr n.
 template <bool INV, int NI, int NK, int KBS>
 __device__ __forceinline__ void stage32_f64(const Blk4d& zr, const Blk4d& zi, const double* __restrict__ sF, int lane,
                                             Blk4d& outr, Blk4d& outi) {
@@ -322,24 +325,23 @@ __device__ __forceinline__ void stage32_f64(const Blk4d& zr, const Blk4d& zi, co
       for (int v = 0; v < 4; v++) sum[R][v] = zr.b[2 * R + I][v] + zi.b[2 * R + I][v];
 #pragma unroll
     for (int Kb = 0; Kb < NK; Kb++) {
-      d4 k1 = {0.0, 0.0, 0.0, 0.0}, k2 = k1, k3 = k1;
+      d4 k1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int R = 0; R < 2; R++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) k1 = FFS_MFMA64(sum[R][v], sF[(((R * 4 + v) * KBS + Kb) * 3) * kLanes + lane], k1);
+      d4 re = k1, im = k1;
 #pragma unroll
       for (int R = 0; R < 2; R++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const double* f = sF + (((R * 4 + v) * KBS + Kb) * 3) * kLanes + lane;
-          const double fr = f[0], fs = f[kLanes], fd = f[2 * kLanes];
-          k1 = FFS_MFMA64(sum[R][v], fr, k1);
-          k2 = FFS_MFMA64(zr.b[2 * R + I][v], INV ? fs : fd, k2);
-          k3 = FFS_MFMA64(zi.b[2 * R + I][v], INV ? fd : fs, k3);
+          const double fn = f[kLanes], fd = f[2 * kLanes];
+          re = FFS_MFMA64(zi.b[2 * R + I][v], INV ? fd : fn, re);
+          im = FFS_MFMA64(zr.b[2 * R + I][v], INV ? fn : fd, im);
         }
-      if (!INV) {
-        outr.b[2 * I + Kb] = k1 - k3;
-        outi.b[2 * I + Kb] = k1 + k2;
-      } else {
-        outr.b[2 * I + Kb] = k1 + k3;
-        outi.b[2 * I + Kb] = k1 - k2;
-      }
+      outr.b[2 * I + Kb] = re;
+      outi.b[2 * I + Kb] = im;
     }
   }
 }
@@ -1695,7 +1697,7 @@ __global__ void k_gen_inverse32_f64(double* out, int32_t shift) {
   const int lane = idx % kLanes, arr = (idx / kLanes) % 3, v = (idx / (3 * kLanes)) % 4, R = idx / (12 * kLanes);
   double fr, fi;
   dft_entry(16 * R + 4 * v + (lane >> 4), ((lane & 15) + shift) & 31, 32, fr, fi);
-  out[idx] = arr == 0 ? fr : (arr == 1 ? fr + fi : fi - fr);
+  out[idx] = arr == 0 ? fr : (arr == 1 ? -(fr + fi) : fi - fr);
 }
 
 // 32-point FP64 B operands (layout kOps32F64): one thread per entry.
@@ -1706,7 +1708,7 @@ __global__ void k_gen_operands32_f64(double* t) {
   const int v = (idx / (6 * kLanes)) % 4, R = idx / (24 * kLanes);
   double fr, fi;
   dft_entry(16 * R + 4 * v + (lane >> 4), 16 * Kb + (lane & 15), 32, fr, fi);
-  t[idx] = arr == 0 ? fr : (arr == 1 ? fr + fi : fi - fr);
+  t[idx] = arr == 0 ? fr : (arr == 1 ? -(fr + fi) : fi - fr);
 }
 
 // K(k) = sum_taps w[a] exp(-2 pi i sum_ax (d_ax(a) k_ax mod L_ax)/L_ax),  d = a (ref) or r - a (centred);
