@@ -14,7 +14,7 @@ enum TileKind : int32_t {
 };
 
 // Operand-table layout in doubles (one table per plan, generated on the GPU in FP64):
-//   DFT B-operands  [axis 0..2][arr: R, S = Fr+Fi, D = Fi-Fr][chunk 0..3][lane 0..63]
+//   DFT B-operands  [axis 0..2][arr: R = Fr, S = -(Fr+Fi), D = Fi-Fr][chunk 0..3][lane 0..63]
 //   spectrum        [main|tail][...] laid out per tile family, see spectrum_len()
 constexpr int kLanes = 64;
 constexpr int kOpsPerAxis = 3 * 4 * kLanes;  // 768 doubles

@@ -86,24 +86,24 @@ __device__ __forceinline__ void load_ops(OpsAxis& o, const double* __restrict__ 
   }
 }
 
-// out = A^T-chain stage: A given as 4 chunks (A operand values), B = F (or conj F when INV).
+// out = A^T-chain stage: A given as 4 chunks (A operand values), B = F (or conj F when INV).  Gauss form
+// on the tables r = Fr, s = -(Fr + Fi), d = Fi - Fr: k1 = (ar + ai) r first, then both outputs chain from
+// it (C input k1, result a new accumulator), so the stage needs no output additions:
+//   forward  Re = k1 + ai s, Im = k1 + ar d;   inverse  Re = k1 + ai d, Im = k1 + ar s.
 template <bool INV>
 __device__ __forceinline__ void cstage(const double (&ar)[4], const double (&ai)[4], const OpsAxis& f, d4& outr,
                                        d4& outi) {
-  d4 k1 = {0.0, 0.0, 0.0, 0.0}, k2 = k1, k3 = k1;
+  d4 k1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 4; c++) k1 = FFS_MFMA64(ar[c] + ai[c], f.r[c], k1);
+  d4 re = k1, im = k1;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    k1 = FFS_MFMA64(ar[c] + ai[c], f.r[c], k1);
-    k2 = FFS_MFMA64(ar[c], INV ? f.s[c] : f.d[c], k2);
-    k3 = FFS_MFMA64(ai[c], INV ? f.d[c] : f.s[c], k3);
+    re = FFS_MFMA64(ai[c], INV ? f.d[c] : f.s[c], re);
+    im = FFS_MFMA64(ar[c], INV ? f.s[c] : f.d[c], im);
   }
-  if (!INV) {
-    outr = k1 - k3;
-    outi = k1 + k2;
-  } else {
-    outr = k1 + k3;
-    outi = k1 - k2;
-  }
+  outr = re;
+  outi = im;
 }
 
 __device__ __forceinline__ void to_chunks(const d4& v, double (&c)[4]) {
@@ -1570,7 +1570,7 @@ __global__ void k_gen_operands(double* ops, int32_t L0, int32_t L1, int32_t L2) 
   }
   double* base = ops + axis * kOpsPerAxis;
   base[(0 * 4 + c) * kLanes + lane] = fr;
-  base[(1 * 4 + c) * kLanes + lane] = fr + fi;
+  base[(1 * 4 + c) * kLanes + lane] = -(fr + fi);
   base[(2 * 4 + c) * kLanes + lane] = fi - fr;
 }
 
