@@ -1090,6 +1090,27 @@ __device__ __forceinline__ void split8_f16(const f4& x0, const f4& x1, h8& h, h8
   l = as_h8(lw);
 }
 
+// split8_f16 of (x0 | x1) * sc for a power-of-two sc, with the scaling folded into the mixed FMAs:
+// h = f16(x sc) and l = f16(x sc - h), each one v_fma_mix with a single rounding.  x sc is exact in FP32
+// (a power-of-two scale of a normal value), so the words equal split8_f16 of the scaled values.
+__device__ __forceinline__ void split8_f16_scaled(const f4& x0, const f4& x1, float sc, h8& h, h8& l) {
+  u4 hw, lw;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const f4& x = k < 2 ? x0 : x1;
+    const int e = 2 * (k & 1);
+    uint32_t w, r;
+    asm("v_fma_mixlo_f16 %0, %1, %2, 0" : "=v"(w) : "v"(x[e]), "v"(sc));
+    asm("v_fma_mixhi_f16 %0, %1, %2, 0" : "+v"(w) : "v"(x[e + 1]), "v"(sc));
+    asm("v_fma_mixlo_f16 %0, %1, %2, -%3 op_sel_hi:[0,0,1]" : "=v"(r) : "v"(x[e]), "v"(sc), "v"(w));
+    asm("v_fma_mixhi_f16 %0, %1, %2, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(r) : "v"(x[e + 1]), "v"(sc), "v"(w));
+    hw[k] = w;
+    lw[k] = r;
+  }
+  h = as_h8(hw);
+  l = as_h8(lw);
+}
+
 // Maximum of a non-negative v over the wave, in every lane: DPP exchanges inside each row of 16 lanes
 // (quad swaps, then the half-row and row mirrors), then the four row maxima through readlane.  For
 // non-negative floats the bit patterns order like unsigned integers, so the max is taken on those.
@@ -1115,16 +1136,22 @@ __device__ __forceinline__ float wave_max_nonneg(float v) {
   return __uint_as_float(max(max(r0, r1), max(r2, r3)));
 }
 
-template <bool INV, int NI = 2, int NK = 2, int KBS = 2>
+// SCALED: the inputs are split as z * sc (the first forward stage, which applies the tile's scale).
+template <bool INV, int NI = 2, int NK = 2, int KBS = 2, bool SCALED = false>
 __device__ __forceinline__ void stage32_f16(const Blk4& zr, const Blk4& zi, const u4* __restrict__ sF, int lane,
-                                            Blk4& outr, Blk4& outi) {
+                                            Blk4& outr, Blk4& outi, float sc = 1.0f) {
 #pragma unroll
   for (int I = 0; I < NI; I++) {
     h8 rh, rl, ih, il;
     f4 x0 = zr.b[I], x1 = zr.b[2 + I], y0 = zi.b[I], y1 = zi.b[2 + I];
     mfma_settle(x0, x1, y0, y1);
-    split8_f16(x0, x1, rh, rl);
-    split8_f16(y0, y1, ih, il);
+    if (SCALED) {
+      split8_f16_scaled(x0, x1, sc, rh, rl);
+      split8_f16_scaled(y0, y1, sc, ih, il);
+    } else {
+      split8_f16(x0, x1, rh, rl);
+      split8_f16(y0, y1, ih, il);
+    }
 #pragma unroll
     for (int Kb = 0; Kb < NK; Kb++) {
       h8 Fr[2], Fi[2], Fn[2];
@@ -1178,15 +1205,7 @@ __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, c
   int k;
   (void)frexpf(m, &k);  // m < 2^k
   const int e = __builtin_amdgcn_readfirstlane(min(max(a.scale_exp - k, -120), 120));
-  const float sc = ldexpf(1.0f, e);
-#pragma unroll
-  for (int b = 0; b < 4; b++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      zr.b[b][r] *= sc;
-      zi.b[b][r] *= sc;
-    }
-  stage32_f16<false>(zr, zi, sF, lane, orr, oi);
+  stage32_f16<false, 2, 2, 2, true>(zr, zi, sF, lane, orr, oi, ldexpf(1.0f, e));
   stage32_f16<false>(orr, oi, sF, lane, zr, zi);
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
