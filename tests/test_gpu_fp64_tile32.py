@@ -67,3 +67,17 @@ def test_2d_fp64_tile32_matches_tile16_at_scale(ffs, torch_cuda):
         outs.append(d_out)
     rel = float((outs[0] - outs[1]).abs().max() / outs[0].abs().max())
     assert rel <= TOL_FP64, rel
+
+
+@pytest.mark.parametrize("n1", [1 << 22, (1 << 22) + 96])
+@pytest.mark.parametrize("precision", ["fp64", "f16", "bf16"])
+def test_2d_tile32_long_rows(ffs, oracle, torch_cuda, n1, precision):
+    # Rows of 2^22 columns are the widest whose interior windows are read through the 32-bit buffer
+    # offsets; wider rows take the general gather for every window.  Both agree with the oracle.
+    _, r, w, _ = ffs.named_stencil("Heat-2D", seed=3)
+    x = oracle.uniform(4242 + n1, (34, n1))
+    got, info = _run(ffs, torch_cuda, x, r, w, 3, tile=32, precision=precision)
+    assert info["tile"][:2] == [32, 32]
+    ref = oracle.sweep(x, r, w, 3)
+    tol = TOL_FP64 if precision == "fp64" else 1e-6
+    assert oracle.errors(got, ref)[1] <= tol, (precision, n1, info)
