@@ -270,9 +270,11 @@ ffs_status choose_config(ffs_plan* pl) {
     const int64_t tmax = r == 0 ? (int64_t)1 << 40 : (Pm - vmin) / (2 * r);
     if (tmax < 1 || (p.fuse > 0 && p.fuse > tmax)) return -1.0;
     double compute = p.precision == FFS_PREC_FP64 ? 4700.0 : (p.precision == FFS_PREC_MIXED_F32 ? 2600.0 : 2500.0);
+    // 32-point pairs (full stages; SIMD cycles per pair from GRBM_GUI_ACTIVE of the reduced-stage kernels
+    // on Box2D25P 8192^2, divided by their factor below): FP16 5550 / 0.63, BF16 8000 / 0.70
     if (kind == ffsk::KIND_2D32)
       compute = p.precision == FFS_PREC_MIXED_BF16 ? 12000.0
-                : p.precision == FFS_PREC_MIXED_F16 ? 3000.0
+                : p.precision == FFS_PREC_MIXED_F16 ? 8800.0
                 : p.precision == FFS_PREC_FP64     ? 31000.0  // 384 FP64 MFMAs per pair
                                                    : 9000.0;
     // 32-point family with V <= 16: reduced inverse stages, measured at 0.63 (FP16), 0.70 (BF16) and
@@ -385,7 +387,7 @@ ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, boo
   a.precision = p.precision;
   a.scale_exp = tail ? pl->scale_tail : pl->scale_main;
   a.linear_map = p.mapping == FFS_MAP_XCD ? 0 : 1;  // AUTO = hardware order (measured 1-3% faster)
-  a.out16 = out16_block(pl) ? 1 : 0;
+  a.out16 = !out16_block(pl) ? 0 : (h[0] == 16 && h[1] == 16 ? 2 : 1);
   if (a.out16) {
     const size_t per = pl->prob.precision == FFS_PREC_FP64 ? sizeof(double) * ffsk::kInv32F64
                                                             : sizeof(uint32_t) * ffsk::kInv32Words;

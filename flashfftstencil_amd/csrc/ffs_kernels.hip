@@ -440,7 +440,9 @@ __device__ __forceinline__ void store16_f64(const TileArgs& a, const PairCoord& 
 
 // One tile pair of the 32-point FP64 family; the spectrum comes from LDS (single-grid launch) or from
 // the item's FP64 table in global memory (batch launch).
-template <bool H_FROM_LDS, bool OUT16>
+// IKBS: column blocks per chunk of the inverse-operand table sG (1: the shifted one-block table; 2: the
+// forward operands offset to column block 1, the inverse operands of shift 16).
+template <bool H_FROM_LDS, bool OUT16, int IKBS = 1>
 __device__ __forceinline__ void tile32_pair_f64(const TileArgs& a, int64_t tp, const double* __restrict__ sF,
                                                 const double2* __restrict__ sH, const double* __restrict__ sG,
                                                 int lane, int q, int j) {
@@ -463,8 +465,8 @@ __device__ __forceinline__ void tile32_pair_f64(const TileArgs& a, int64_t tp, c
       zi.b[b][v] = yi;
     }
   if (OUT16) {
-    stage32_f64<true, 2, 1, 1>(zr, zi, sG, lane, orr, oi);  // blocks 0 and 2: window rows h0 ..
-    stage32_f64<true, 1, 1, 1>(orr, oi, sG, lane, zr, zi);  // block 0 = the valid region
+    stage32_f64<true, 2, 1, IKBS>(zr, zi, sG, lane, orr, oi);  // blocks 0 and 2: window rows h0 ..
+    stage32_f64<true, 1, 1, IKBS>(orr, oi, sG, lane, zr, zi);  // block 0 = the valid region
     store16_f64(a, pc, q, j, zr.b[0], zi.b[0]);
   } else {
     stage32_f64<true, 2, 2, 2>(zr, zi, sF, lane, orr, oi);
@@ -509,7 +511,9 @@ __global__ __launch_bounds__(256, 2) void k_batch32_f64(const TileArgs* __restri
       if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
     const TileArgs a = items[lo];
-    if (a.out16)
+    if (a.out16 == 2)
+      tile32_pair_f64<false, true, 2>(a, idx - prefix[lo], sF, nullptr, sF + 3 * kLanes, lane, q, j);
+    else if (a.out16)
       tile32_pair_f64<false, true>(a, idx - prefix[lo], sF, nullptr, static_cast<const double*>(a.opsi), lane, q, j);
     else
       tile32_pair_f64<false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
@@ -1000,7 +1004,7 @@ __device__ __forceinline__ void store16(const TileArgs& a, const PairCoord& pc, 
 
 // One tile pair of the 32-point family.  The spectrum comes from LDS (single-grid launch) or from the
 // grid's FP64 table in global memory (batch launch, where every grid has its own stencil and depth).
-template <int NTERMS, bool H_FROM_LDS, bool OUT16>
+template <int NTERMS, bool H_FROM_LDS, bool OUT16, int IKBS = 1>  // IKBS as for tile32_pair_f64
 __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const u4* __restrict__ sF,
                                             const f2* __restrict__ sH, const u4* __restrict__ sG, int lane, int q,
                                             int j) {
@@ -1035,8 +1039,8 @@ __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const
       zi.b[b][v] = yi;
     }
   if (OUT16) {
-    stage32<true, NTERMS, 2, 1, 1>(zr, zi, sG, lane, orr, oi);  // blocks 0 and 2: window rows h0 ..
-    stage32<true, NTERMS, 1, 1, 1>(orr, oi, sG, lane, zr, zi);  // block 0 = the valid region
+    stage32<true, NTERMS, 2, 1, IKBS>(zr, zi, sG, lane, orr, oi);  // blocks 0 and 2: window rows h0 ..
+    stage32<true, NTERMS, 1, 1, IKBS>(orr, oi, sG, lane, zr, zi);  // block 0 = the valid region
     store16(a, pc, q, j, zr.b[0], zi.b[0], cA * a.dc_gain, cB * a.dc_gain);
   } else {
     stage32<true, NTERMS>(zr, zi, sF, lane, orr, oi);
@@ -1183,7 +1187,7 @@ __device__ __forceinline__ void stage32_f16(const Blk4& zr, const Blk4& zi, cons
   }
 }
 
-template <bool H_FROM_LDS, bool OUT16>
+template <bool H_FROM_LDS, bool OUT16, int IKBS = 1>  // IKBS as for tile32_pair_f64
 __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, const u4* __restrict__ sF,
                                                 const f2* __restrict__ sH, const u4* __restrict__ sG, int lane,
                                                 int q, int j) {
@@ -1221,8 +1225,8 @@ __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, c
       zi.b[b][v] = yi;
     }
   if (OUT16) {
-    stage32_f16<true, 2, 1, 1>(zr, zi, sG, lane, orr, oi);
-    stage32_f16<true, 1, 1, 1>(orr, oi, sG, lane, zr, zi);
+    stage32_f16<true, 2, 1, IKBS>(zr, zi, sG, lane, orr, oi);
+    stage32_f16<true, 1, 1, IKBS>(orr, oi, sG, lane, zr, zi);
     store16(a, pc, q, j, zr.b[0], zi.b[0], cA * a.dc_gain, cB * a.dc_gain, ldexp(1.0, -e));
   } else {
     stage32_f16<true>(zr, zi, sF, lane, orr, oi);
@@ -1289,7 +1293,9 @@ __global__ __launch_bounds__(256, 3) void k_batch32_bf16(const TileArgs* __restr
       if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
     const TileArgs a = items[lo];
-    if (a.out16)
+    if (a.out16 == 2)
+      tile32_pair<NTERMS, false, true, 2>(a, idx - prefix[lo], sF, nullptr, sF + 3 * kLanes, lane, q, j);
+    else if (a.out16)
       tile32_pair<NTERMS, false, true>(a, idx - prefix[lo], sF, nullptr, static_cast<const u4*>(a.opsi), lane, q, j);
     else
       tile32_pair<NTERMS, false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
@@ -1331,7 +1337,9 @@ __global__ __launch_bounds__(256, 3) void k_batch32_f16(const TileArgs* __restri
       if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
     const TileArgs a = items[lo];
-    if (a.out16)
+    if (a.out16 == 2)
+      tile32_pair_f16<false, true, 2>(a, idx - prefix[lo], sF, nullptr, sF + 3 * kLanes, lane, q, j);
+    else if (a.out16)
       tile32_pair_f16<false, true>(a, idx - prefix[lo], sF, nullptr, static_cast<const u4*>(a.opsi), lane, q, j);
     else
       tile32_pair_f16<false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
