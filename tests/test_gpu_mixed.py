@@ -196,3 +196,15 @@ def test_f16_form_is_2d_only(ffs, torch_cuda):
         ffs.Plan(1, (100,), np.array([0.25, 0.5, 0.25]), 1, 4, precision="f16")
     with pytest.raises(ffs.FFSError, match="UNSUPPORTED"):
         ffs.Plan(2, (64, 64), np.full(9, 1 / 9), 1, 4, precision="f16", tile=16)
+
+
+@pytest.mark.parametrize("form", ["f16", "bf16", "fp64"])
+@pytest.mark.parametrize("shape,depth", [("Heat-2D", 8), ("Box2D9P", 8), ("Box2D25P", 4)])
+def test_planner_depth_8192(ffs, torch_cuda, form, shape, depth):
+    # The cost model's choice on the 8192^2 workloads: the deepest fusion whose valid region is one
+    # 16 x 16 block of the 32-point window (V = 32 - 2 r t = 16), not a deeper one with fewer valid points.
+    _, r, w, _ = ffs.named_stencil(shape, seed=1)
+    plan = ffs.Plan(2, (8192, 8192), w, r, 256, precision=form)
+    info = plan.info
+    plan.close()
+    assert info["tile"][:2] == [32, 32] and info["fuse"] == depth and info["valid"][:2] == [16, 16], info
