@@ -387,7 +387,7 @@ ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, boo
   a.precision = p.precision;
   a.scale_exp = tail ? pl->scale_tail : pl->scale_main;
   a.linear_map = p.mapping == FFS_MAP_XCD ? 0 : 1;  // AUTO = hardware order (measured 1-3% faster)
-  a.out16 = !out16_block(pl) ? 0 : (h[0] == 16 && h[1] == 16 ? 2 : 1);
+  a.out16 = out16_block(pl) ? 1 : 0;
   if (a.out16) {
     const size_t per = pl->prob.precision == FFS_PREC_FP64 ? sizeof(double) * ffsk::kInv32F64
                                                             : sizeof(uint32_t) * ffsk::kInv32Words;

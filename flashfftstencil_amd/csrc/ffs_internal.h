@@ -64,9 +64,8 @@ struct TileArgs {
   // 32-point family: 1 when the valid region fits one 16 x 16 block (V0, V1 <= 16).  The two inverse
   // stages then compute only output block 0 (2 and 1 of 4 output blocks) with the inverse operands of
   // window columns h .. h + 15 (`opsi`), so output index m is window position h + m: every value kept is
-  // the one the full stages would produce at that position, from the same operands.  2 when, in
-  // addition, h = 16 in both axes: those inverse operands are column block 1 of the forward operands, which
-  // the batch kernels already hold in LDS (they read `opsi` from memory otherwise).
+  // the one the full stages would produce at that position, from the same operands (the batch kernels
+  // read those columns from the forward operands they hold in LDS instead of `opsi`).
   int32_t out16;
   const void* opsi;  // out16: inverse operands (kInv32Words words or kInv32F64 doubles, see below)
 };

@@ -440,8 +440,19 @@ __device__ __forceinline__ void store16_f64(const TileArgs& a, const PairCoord& 
 
 // One tile pair of the 32-point FP64 family; the spectrum comes from LDS (single-grid launch) or from
 // the item's FP64 table in global memory (batch launch).
+// The inverse operands of shift h (output column m <- window column (m + h) mod 32) are columns of the
+// forward operand table: lane j + 16q of column block 0 of the shifted table holds what lane
+// (c & 15) + 16q of column block c >> 4 holds in the forward one, c = (j + h) mod 32.  Returns the
+// per-lane base that makes a KBS = 2 read of block 0 at this lane land there; `blk` is the distance
+// between column blocks in the table's elements (3 * kLanes for both layouts).
+template <typename T>
+__device__ __forceinline__ const T* shifted_view(const T* table, int h, int lane) {
+  const int c = ((lane & 15) + h) & 31;
+  return table + (c >> 4) * 3 * kLanes + ((lane & 48) | (c & 15)) - lane;
+}
+
 // IKBS: column blocks per chunk of the inverse-operand table sG (1: the shifted one-block table; 2: the
-// forward operands offset to column block 1, the inverse operands of shift 16).
+// forward operand table, read through shifted_view).
 template <bool H_FROM_LDS, bool OUT16, int IKBS = 1>
 __device__ __forceinline__ void tile32_pair_f64(const TileArgs& a, int64_t tp, const double* __restrict__ sF,
                                                 const double2* __restrict__ sH, const double* __restrict__ sG,
@@ -511,10 +522,8 @@ __global__ __launch_bounds__(256, 2) void k_batch32_f64(const TileArgs* __restri
       if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
     const TileArgs a = items[lo];
-    if (a.out16 == 2)
-      tile32_pair_f64<false, true, 2>(a, idx - prefix[lo], sF, nullptr, sF + 3 * kLanes, lane, q, j);
-    else if (a.out16)
-      tile32_pair_f64<false, true>(a, idx - prefix[lo], sF, nullptr, static_cast<const double*>(a.opsi), lane, q, j);
+    if (a.out16)
+      tile32_pair_f64<false, true, 2>(a, idx - prefix[lo], sF, nullptr, shifted_view(sF, a.h1, lane), lane, q, j);
     else
       tile32_pair_f64<false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
   }
@@ -1293,10 +1302,8 @@ __global__ __launch_bounds__(256, 3) void k_batch32_bf16(const TileArgs* __restr
       if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
     const TileArgs a = items[lo];
-    if (a.out16 == 2)
-      tile32_pair<NTERMS, false, true, 2>(a, idx - prefix[lo], sF, nullptr, sF + 3 * kLanes, lane, q, j);
-    else if (a.out16)
-      tile32_pair<NTERMS, false, true>(a, idx - prefix[lo], sF, nullptr, static_cast<const u4*>(a.opsi), lane, q, j);
+    if (a.out16)
+      tile32_pair<NTERMS, false, true, 2>(a, idx - prefix[lo], sF, nullptr, shifted_view(sF, a.h1, lane), lane, q, j);
     else
       tile32_pair<NTERMS, false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
   }
@@ -1337,10 +1344,8 @@ __global__ __launch_bounds__(256, 3) void k_batch32_f16(const TileArgs* __restri
       if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
     const TileArgs a = items[lo];
-    if (a.out16 == 2)
-      tile32_pair_f16<false, true, 2>(a, idx - prefix[lo], sF, nullptr, sF + 3 * kLanes, lane, q, j);
-    else if (a.out16)
-      tile32_pair_f16<false, true>(a, idx - prefix[lo], sF, nullptr, static_cast<const u4*>(a.opsi), lane, q, j);
+    if (a.out16)
+      tile32_pair_f16<false, true, 2>(a, idx - prefix[lo], sF, nullptr, shifted_view(sF, a.h1, lane), lane, q, j);
     else
       tile32_pair_f16<false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
   }

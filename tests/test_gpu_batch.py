@@ -48,15 +48,18 @@ def test_batch_matches_oracle_and_single_plans(ffs, oracle, torch_cuda, form):
 
 @pytest.mark.parametrize("form", ["bf16", "f16", "fp64"])
 def test_batch_mixes_valid_widths(ffs, oracle, torch_cuda, form):
-    # one launch holds tile pairs with V <= 16 (reduced inverse stages) and V > 16 (full ones)
+    # one launch holds tile pairs with V <= 16 (reduced inverse stages at window offsets h = 8, 12, 16 and
+    # 18) and V > 16 (full ones), in both index conventions
     torch = torch_cuda
     specs = []
-    for i, (r, fuse, sizes, steps) in enumerate([(2, 4, (70, 45), 9), (2, 3, (33, 61), 7), (1, 8, (50, 50), 17),
-                                                 (1, 5, (40, 29), 11), (3, 3, (64, 64), 6)]):
+    for i, (r, fuse, sizes, steps, conv) in enumerate([(2, 4, (70, 45), 9, "ref"), (2, 3, (33, 61), 7, "ref"),
+                                                       (1, 8, (50, 50), 17, "ref"), (1, 5, (40, 29), 11, "ref"),
+                                                       (3, 3, (64, 64), 6, "ref"), (2, 4, (45, 70), 9, "centred"),
+                                                       (3, 4, (37, 52), 8, "centred")]):
         rng = np.random.default_rng(300 + i)
         w = rng.random((2 * r + 1) ** 2)
         w /= w.sum()
-        specs.append(dict(radius=r, weights=w, sizes=sizes, steps=steps, fuse=fuse))
+        specs.append(dict(radius=r, weights=w, sizes=sizes, steps=steps, fuse=fuse, convention=conv))
     xs = [oracle.uniform(500 + i, s["sizes"]) for i, s in enumerate(specs)]
     d_in = [torch.from_numpy(x).cuda() for x in xs]
     d_out = [torch.empty_like(d) for d in d_in]
@@ -64,9 +67,10 @@ def test_batch_mixes_valid_widths(ffs, oracle, torch_cuda, form):
     batch.run(d_in, d_out)
     torch.cuda.synchronize()
     for i, s in enumerate(specs):
-        ref = oracle.sweep(xs[i], s["radius"], s["weights"], s["steps"])
+        ref = oracle.sweep(xs[i], s["radius"], s["weights"], s["steps"], convention=0 if s["convention"] == "ref" else 1)
         assert oracle.errors(d_out[i].cpu().numpy(), ref)[1] <= (1e-12 if form == "fp64" else 1e-6), (i, batch.info(i))
-        plan = ffs.Plan(2, s["sizes"], s["weights"], s["radius"], s["steps"], precision=form, fuse=s["fuse"], tile=32)
+        plan = ffs.Plan(2, s["sizes"], s["weights"], s["radius"], s["steps"], precision=form, fuse=s["fuse"], tile=32,
+                        convention=s["convention"])
         single = torch.empty_like(d_in[i])
         plan.run(d_in[i], single)
         torch.cuda.synchronize()
