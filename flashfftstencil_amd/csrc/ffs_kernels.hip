@@ -1435,6 +1435,48 @@ struct Prec3<PREC_F32, NTERMS> {
   static __device__ __forceinline__ void load_ops(Ops& f, const TileArgs& a, int lane) { load_ops_mixed(f, a.opsm, 0, lane); }
 };
 
+// Geometry of 3D tile pair tp: tile origins (o*), window origins (w*), whether tile B exists and
+// whether the window needs no periodic wrap.
+struct Win3 {
+  int64_t o0, o1, o2A, o2B, w0, w1, w2A, w2B;
+  bool hasB, interior;
+};
+
+__device__ __forceinline__ Win3 win3(const TileArgs& a, int64_t tp) {
+  Win3 g;
+  const int64_t pairs2 = (a.tiles2 + 1) >> 1;
+  const int64_t T0 = tp / (a.tiles1 * pairs2);
+  const int64_t rem = tp - T0 * (a.tiles1 * pairs2);
+  const int64_t T1 = rem / pairs2, T2 = (rem - T1 * pairs2) * 2;
+  g.hasB = T2 + 1 < a.tiles2;
+  g.o0 = T0 * a.V0;
+  g.o1 = T1 * a.V1;
+  g.o2A = T2 * a.V2;
+  g.o2B = g.o2A + a.V2;
+  g.w0 = g.o0 - a.h0;
+  g.w1 = g.o1 - a.h1;
+  g.w2A = g.o2A - a.h2;
+  g.w2B = g.o2B - a.h2;
+  g.interior = g.w0 >= 0 && g.w0 + 16 <= a.n0 && g.w1 >= 0 && g.w1 + 16 <= a.n1 && g.w2A >= 0 &&
+               (g.hasB ? g.w2B : g.w2A) + 16 <= a.n2;
+  return g;
+}
+
+// Window values of thread (y, z): slice x, row y, column z of tile A (va) and tile B (vb, 0 without B).
+__device__ __forceinline__ void load3(const TileArgs& a, const Win3& g, int y, int z, double (&va)[16],
+                                      double (&vb)[16]) {
+  const int64_t r1 = g.interior ? g.w1 + y : wrap_near(g.w1 + y, a.n1);
+  const int64_t cAz = g.interior ? g.w2A + z : wrap_near(g.w2A + z, a.n2);
+  const int64_t cBz = g.interior ? g.w2B + z : wrap_near(g.w2B + z, a.n2);
+#pragma unroll
+  for (int x = 0; x < 16; x++) {
+    const int64_t r0 = g.interior ? g.w0 + x : wrap_near(g.w0 + x, a.n0);
+    const double* row = a.src + (r0 * a.n1 + r1) * a.n2;
+    va[x] = row[cAz];
+    vb[x] = g.hasB ? row[cBz] : 0.0;
+  }
+}
+
 template <int PREC, int NTERMS>
 __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
   typedef Prec3<PREC, NTERMS> P;
@@ -1447,43 +1489,42 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
   typename P::Ops f;
   P::load_ops(f, a, lane);
   const bool mixed = PREC != 0;
-  const int64_t pairs2 = (a.tiles2 + 1) >> 1;
 
-  for (int64_t tp = block_order(a.linear_map, blockIdx.x, gridDim.x); tp < a.pairs; tp += gridDim.x) {
-    const int64_t T0 = tp / (a.tiles1 * pairs2);
-    const int64_t rem = tp - T0 * (a.tiles1 * pairs2);
-    const int64_t T1 = rem / pairs2, T2 = (rem - T1 * pairs2) * 2;
-    const bool hasB = T2 + 1 < a.tiles2;
-    const int64_t o0 = T0 * a.V0, o1 = T1 * a.V1, o2A = T2 * a.V2, o2B = o2A + a.V2;
-    const int64_t w0 = o0 - a.h0, w1 = o1 - a.h1, w2A = o2A - a.h2, w2B = o2B - a.h2;
-    const bool interior = w0 >= 0 && w0 + 16 <= a.n0 && w1 >= 0 && w1 + 16 <= a.n1 && w2A >= 0 &&
-                          (hasB ? w2B : w2A) + 16 <= a.n2;
+  // Window of the pair: every thread (y = tid / 16, z = tid % 16) loads column z of row y in all 16
+  // slices x, 32 loads in flight.  With PREFETCH (FP64) the next pair's window is loaded right after
+  // the current one is in LDS, so its latency hides behind the three compute phases (64 VGPRs, within
+  // the 2 waves per SIMD the 70 KB of LDS allow anyway).
+  constexpr bool PREFETCH = PREC == 0;
+  const int fy = threadIdx.x >> 4, fz = threadIdx.x & 15;
+  double va[16], vb[16];
+  int64_t tp = block_order(a.linear_map, blockIdx.x, gridDim.x);
+  Win3 g = win3(a, tp);
+  if (tp < a.pairs) load3(a, g, fy, fz, va, vb);
+  for (; tp < a.pairs; tp += gridDim.x) {
+    if (!PREFETCH) load3(a, g, fy, fz, va, vb);
+    const Win3 cur = g;
+    const int64_t o0 = cur.o0, o1 = cur.o1, o2A = cur.o2A, o2B = cur.o2B;
+    const bool hasB = cur.hasB;
     // DC offsets (mixed forms): the window's center sample (8, 8, 8), a wave-uniform load
     double cA = 0.0, cB = 0.0;
     if (mixed) {
-      const int64_t base = (wrap_near(w0 + 8, a.n0) * a.n1 + wrap_near(w1 + 8, a.n1)) * a.n2;
-      cA = a.src[base + wrap_near(w2A + 8, a.n2)];
-      cB = hasB ? a.src[base + wrap_near(w2B + 8, a.n2)] : 0.0;
+      const int64_t base = (wrap_near(cur.w0 + 8, a.n0) * a.n1 + wrap_near(cur.w1 + 8, a.n1)) * a.n2;
+      cA = a.src[base + wrap_near(cur.w2A + 8, a.n2)];
+      cB = hasB ? a.src[base + wrap_near(cur.w2B + 8, a.n2)] : 0.0;
     }
-    // ---- fill: thread (y = tid / 16, z = tid % 16) loads column z of row y in every slice x ----
-    {
-      const int y = threadIdx.x >> 4, z = threadIdx.x & 15;
-      const int64_t r1 = interior ? w1 + y : wrap_near(w1 + y, a.n1);
-      const int64_t cAz = interior ? w2A + z : wrap_near(w2A + z, a.n2);
-      const int64_t cBz = interior ? w2B + z : wrap_near(w2B + z, a.n2);
-#pragma unroll 4
-      for (int x = 0; x < 16; x++) {
-        const int64_t r0 = interior ? w0 + x : wrap_near(w0 + x, a.n0);
-        const double* row = a.src + (r0 * a.n1 + r1) * a.n2;
-        const double va = row[cAz];
-        const double vb = hasB ? row[cBz] : 0.0;
-        C2 c;
-        c.re = (V)(va - cA);
-        c.im = (V)(hasB ? vb - cB : 0.0);
-        sX[lds3(x, y, z)] = c;
-      }
+#pragma unroll
+    for (int x = 0; x < 16; x++) {
+      C2 c;
+      c.re = (V)(va[x] - cA);
+      c.im = (V)(hasB ? vb[x] - cB : 0.0);
+      sX[lds3(x, fy, fz)] = c;
     }
     __syncthreads();
+    const int64_t tn = tp + gridDim.x;
+    if (tn < a.pairs) {
+      g = win3(a, tn);
+      if (PREFETCH) load3(a, g, fy, fz, va, vb);
+    }
     // ---- phase 1: slices a = 4w + s; contract axis 2, then axis 1 (two slices in flight) ----
 #pragma unroll 2
     for (int s = 0; s < 4; s++) {
