@@ -262,10 +262,13 @@ ffs_status choose_config(ffs_plan* pl) {
   }
   const int vmin = p.dims == 1 ? 16 : 2;
   const int dd = p.dims == 1 ? 1 : p.dims;
-  // Maximise valid point-steps per unit time of one tile pair: 2 V^d t / max(compute, hbm), where the
-  // compute cost of a pair is fixed by the tile family and precision (cycles per SIMD, measured on
-  // MI355X) and the HBM cost is 16 B per valid point at ~3.26 B/cycle/SIMD (8 TB/s over 1024 SIMDs at
-  // 2.4 GHz).  Returns the score of the best (or the forced) depth of a family, -1 when none fits.
+  // Minimise the modelled time of the whole run: nblocks x (launch + pairs x pair cost / 1024 SIMDs),
+  // with the depth rebalanced as set_blocks does (same block count, shallowest depth; the tail block
+  // shares the main block's tiling, so it costs the same).  The cost of a pair is max(compute, hbm):
+  // compute fixed by the tile family and precision (cycles per SIMD, measured on MI355X), HBM 16 B per
+  // valid point at ~3.26 B/cycle/SIMD (8 TB/s over 1024 SIMDs at 2.4 GHz); a launch adds a fixed ramp-up
+  // and drain (~4 us), which decides the depth only for small grids.  Returns point-steps per modelled
+  // cycle of the best (or the forced) depth of a family, -1 when none fits.
   auto search = [&](int kind, int Pm, int64_t& t_out) -> double {
     const int64_t tmax = r == 0 ? (int64_t)1 << 40 : (Pm - vmin) / (2 * r);
     if (tmax < 1 || (p.fuse > 0 && p.fuse > tmax)) return -1.0;
@@ -281,16 +284,29 @@ ffs_status choose_config(ffs_plan* pl) {
     // 0.74 (FP64) of a full tile pair on MI355X
     const double cut = p.precision == FFS_PREC_FP64 ? 0.74 : 0.65;
     const double hbm_per_point = 2.0 * 16.0 / 3.26;
+    const double launch = 8000.0;
+    const int64_t T = std::max<int64_t>(p.steps, 1);
+    double points = 1.0;
+    for (int d = 0; d < p.dims; d++) points *= (double)p.sizes[d];
     double best = -1;
     const int64_t t0 = p.fuse > 0 ? p.fuse : 1, t1 = p.fuse > 0 ? p.fuse : std::min<int64_t>(tmax, 4096);
     for (int64_t t = t0; t <= t1; t++) {
-      const double Vd = std::pow(Pm - 2.0 * r * t, dd);
-      const bool out16 = kind == ffsk::KIND_2D32 && Pm - 2 * r * t <= 16;
+      const int64_t nb = (T + t - 1) / t;
+      const int64_t te = p.fuse > 0 ? t : (T + nb - 1) / nb;  // the depth set_blocks runs
+      const int64_t V = Pm - 2 * r * te;
+      const double Vd = std::pow((double)V, dd);
+      double pairs = 1.0;  // tiles per axis, pairs along the last one
+      for (int d = 0; d < p.dims; d++) {
+        const double tiles = std::ceil((double)p.sizes[d] / (double)V);
+        pairs *= d == p.dims - 1 ? std::ceil(tiles / 2.0) : tiles;
+      }
+      const bool out16 = kind == ffsk::KIND_2D32 && V <= 16;
       const double c = compute * (p.dims == 3 ? 16.0 : 1.0) * (out16 ? cut : 1.0);
-      const double score = Vd * t / std::max(c, hbm_per_point * Vd);
+      const double cycles = (double)nb * (launch + pairs * std::max(c, hbm_per_point * Vd) / 1024.0);
+      const double score = points * (double)T / cycles;
       if (score > best * (1 + 1e-9)) {
         best = score;
-        t_out = t;
+        t_out = te;
       }
     }
     return best;

@@ -206,3 +206,13 @@ def test_fill_uniform_device_matches_host(ffs, oracle, torch_cuda):
     ffs.fill_uniform_device(0x5EED0002, d)
     torch.cuda.synchronize()
     assert np.array_equal(d.cpu().numpy(), ffs.fill_uniform(0x5EED0002, d.numel()))
+
+
+def test_planner_heat1d_single_launch(ffs, torch_cuda):
+    # 2^20 points x 64 steps: the run is launch-bound, and the planner's whole-run model takes the one
+    # block of depth 64 (V = 112 of the 240-point window) over two blocks of depth 32.
+    _, r, w, _ = ffs.named_stencil("Heat-1D", seed=1)
+    plan = ffs.Plan(1, (1 << 20,), w, r, 64, precision="fp64")
+    info = plan.info
+    plan.close()
+    assert info["fuse"] == 64 and info["blocks"] == 1 and info["valid"][0] == 112, info
