@@ -442,21 +442,25 @@ __device__ __forceinline__ void store16_f64(const TileArgs& a, const PairCoord& 
 // the item's FP64 table in global memory (batch launch).
 // The inverse operands of shift h (output column m <- window column (m + h) mod 32) are columns of the
 // forward operand table: lane j + 16q of column block 0 of the shifted table holds what lane
-// (c & 15) + 16q of column block c >> 4 holds in the forward one, c = (j + h) mod 32.  Returns the
-// per-lane base that makes a KBS = 2 read of block 0 at this lane land there; `blk` is the distance
-// between column blocks in the table's elements (3 * kLanes for both layouts).
+// (c & 15) + 16q of column block c >> 4 holds in the forward one, c = (j + h) mod 32.  ShiftedView
+// gives this lane's column block (`base`, read with KBS = 2 as block 0) and the lane to read there.
 template <typename T>
-__device__ __forceinline__ const T* shifted_view(const T* table, int h, int lane) {
+struct ShiftedView {
+  const T* base;
+  int lane;
+};
+template <typename T>
+__device__ __forceinline__ ShiftedView<T> shifted_view(const T* table, int h, int lane) {
   const int c = ((lane & 15) + h) & 31;
-  return table + (c >> 4) * 3 * kLanes + ((lane & 48) | (c & 15)) - lane;
+  return {table + (c >> 4) * 3 * kLanes, (lane & 48) | (c & 15)};
 }
 
 // IKBS: column blocks per chunk of the inverse-operand table sG (1: the shifted one-block table; 2: the
-// forward operand table, read through shifted_view).
+// forward operand table, read through a shifted_view whose lane is `ilane`).
 template <bool H_FROM_LDS, bool OUT16, int IKBS = 1>
 __device__ __forceinline__ void tile32_pair_f64(const TileArgs& a, int64_t tp, const double* __restrict__ sF,
                                                 const double2* __restrict__ sH, const double* __restrict__ sG,
-                                                int lane, int q, int j) {
+                                                int lane, int q, int j, int ilane = -1) {
   const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
   Blk4d zr, zi, orr, oi;
   gather32_f64(a, pc, q, j, zr, zi);
@@ -476,8 +480,9 @@ __device__ __forceinline__ void tile32_pair_f64(const TileArgs& a, int64_t tp, c
       zi.b[b][v] = yi;
     }
   if (OUT16) {
-    stage32_f64<true, 2, 1, IKBS>(zr, zi, sG, lane, orr, oi);  // blocks 0 and 2: window rows h0 ..
-    stage32_f64<true, 1, 1, IKBS>(orr, oi, sG, lane, zr, zi);  // block 0 = the valid region
+    const int il = ilane < 0 ? lane : ilane;  // the lane whose inverse operands this lane reads
+    stage32_f64<true, 2, 1, IKBS>(zr, zi, sG, il, orr, oi);  // blocks 0 and 2: window rows h0 ..
+    stage32_f64<true, 1, 1, IKBS>(orr, oi, sG, il, zr, zi);  // block 0 = the valid region
     store16_f64(a, pc, q, j, zr.b[0], zi.b[0]);
   } else {
     stage32_f64<true, 2, 2, 2>(zr, zi, sF, lane, orr, oi);
@@ -522,10 +527,12 @@ __global__ __launch_bounds__(256, 2) void k_batch32_f64(const TileArgs* __restri
       if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
     const TileArgs a = items[lo];
-    if (a.out16)
-      tile32_pair_f64<false, true, 2>(a, idx - prefix[lo], sF, nullptr, shifted_view(sF, a.h1, lane), lane, q, j);
-    else
+    if (a.out16) {
+      const ShiftedView<double> v = shifted_view(sF, a.h1, lane);
+      tile32_pair_f64<false, true, 2>(a, idx - prefix[lo], sF, nullptr, v.base, lane, q, j, v.lane);
+    } else {
       tile32_pair_f64<false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
+    }
   }
 }
 
@@ -1016,7 +1023,7 @@ __device__ __forceinline__ void store16(const TileArgs& a, const PairCoord& pc, 
 template <int NTERMS, bool H_FROM_LDS, bool OUT16, int IKBS = 1>  // IKBS as for tile32_pair_f64
 __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const u4* __restrict__ sF,
                                             const f2* __restrict__ sH, const u4* __restrict__ sG, int lane, int q,
-                                            int j) {
+                                            int j, int ilane = -1) {
   const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
   double xa[16], xb[16];
   gather32(a, pc, q, j, xa, xb);
@@ -1048,8 +1055,9 @@ __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const
       zi.b[b][v] = yi;
     }
   if (OUT16) {
-    stage32<true, NTERMS, 2, 1, IKBS>(zr, zi, sG, lane, orr, oi);  // blocks 0 and 2: window rows h0 ..
-    stage32<true, NTERMS, 1, 1, IKBS>(orr, oi, sG, lane, zr, zi);  // block 0 = the valid region
+    const int il = ilane < 0 ? lane : ilane;  // the lane whose inverse operands this lane reads
+    stage32<true, NTERMS, 2, 1, IKBS>(zr, zi, sG, il, orr, oi);  // blocks 0 and 2: window rows h0 ..
+    stage32<true, NTERMS, 1, 1, IKBS>(orr, oi, sG, il, zr, zi);  // block 0 = the valid region
     store16(a, pc, q, j, zr.b[0], zi.b[0], cA * a.dc_gain, cB * a.dc_gain);
   } else {
     stage32<true, NTERMS>(zr, zi, sF, lane, orr, oi);
@@ -1199,7 +1207,7 @@ __device__ __forceinline__ void stage32_f16(const Blk4& zr, const Blk4& zi, cons
 template <bool H_FROM_LDS, bool OUT16, int IKBS = 1>  // IKBS as for tile32_pair_f64
 __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, const u4* __restrict__ sF,
                                                 const f2* __restrict__ sH, const u4* __restrict__ sG, int lane,
-                                                int q, int j) {
+                                                int q, int j, int ilane = -1) {
   const PairCoord pc = pair_coord<KIND_2D16>(a, tp);
   double xa[16], xb[16];
   gather32(a, pc, q, j, xa, xb);
@@ -1234,8 +1242,9 @@ __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, c
       zi.b[b][v] = yi;
     }
   if (OUT16) {
-    stage32_f16<true, 2, 1, IKBS>(zr, zi, sG, lane, orr, oi);
-    stage32_f16<true, 1, 1, IKBS>(orr, oi, sG, lane, zr, zi);
+    const int il = ilane < 0 ? lane : ilane;  // the lane whose inverse operands this lane reads
+    stage32_f16<true, 2, 1, IKBS>(zr, zi, sG, il, orr, oi);
+    stage32_f16<true, 1, 1, IKBS>(orr, oi, sG, il, zr, zi);
     store16(a, pc, q, j, zr.b[0], zi.b[0], cA * a.dc_gain, cB * a.dc_gain, ldexp(1.0, -e));
   } else {
     stage32_f16<true>(zr, zi, sF, lane, orr, oi);
@@ -1302,10 +1311,12 @@ __global__ __launch_bounds__(256, 3) void k_batch32_bf16(const TileArgs* __restr
       if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
     const TileArgs a = items[lo];
-    if (a.out16)
-      tile32_pair<NTERMS, false, true, 2>(a, idx - prefix[lo], sF, nullptr, shifted_view(sF, a.h1, lane), lane, q, j);
-    else
+    if (a.out16) {
+      const ShiftedView<u4> v = shifted_view(sF, a.h1, lane);
+      tile32_pair<NTERMS, false, true, 2>(a, idx - prefix[lo], sF, nullptr, v.base, lane, q, j, v.lane);
+    } else {
       tile32_pair<NTERMS, false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
+    }
   }
 }
 
@@ -1344,10 +1355,12 @@ __global__ __launch_bounds__(256, 3) void k_batch32_f16(const TileArgs* __restri
       if (prefix[mid] <= idx) lo = mid; else hi = mid;
     }
     const TileArgs a = items[lo];
-    if (a.out16)
-      tile32_pair_f16<false, true, 2>(a, idx - prefix[lo], sF, nullptr, shifted_view(sF, a.h1, lane), lane, q, j);
-    else
+    if (a.out16) {
+      const ShiftedView<u4> v = shifted_view(sF, a.h1, lane);
+      tile32_pair_f16<false, true, 2>(a, idx - prefix[lo], sF, nullptr, v.base, lane, q, j, v.lane);
+    } else {
       tile32_pair_f16<false, false>(a, idx - prefix[lo], sF, nullptr, nullptr, lane, q, j);
+    }
   }
 }
 
