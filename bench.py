@@ -9,7 +9,7 @@ uniform[0,1), both from SplitMix64 seeded 0x5EED0002.  `value` is the fastest mi
 the plain FP64-MFMA form and the direct FP64 kernel are timed beside it, and all are checked against
 the FP64 direct sweep (the oracle) on a bounded sample.
 
-Printed: one JSON line (see the keys below).  --gpus N > 1 exits: this build runs on one GPU.
+Printed: one JSON line (see the keys below).  --gpus N > 1 exits: the library runs on one GPU.
 """
 import argparse
 import json
@@ -138,7 +138,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        print("bench.py: the library runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -149,7 +149,7 @@ int main(int argc, char** argv) {
   if (ffs_stencil_named(shape.c_str(), seed, &dims, &radius, w.data(), (int32_t)w.size(), tag, sizeof tag) != FFS_OK ||
       dims != kDims) {
     std::cout << "FlashFFTStencil can support stencil calculations for more shapes; '" << shape
-              << "' is not a " << kDims << "D shape of this build." << std::endl;
+              << "' is not one of the named " << kDims << "D shapes." << std::endl;
     return 1;
   }
   const int K = 2 * radius + 1;

@@ -42,7 +42,7 @@ extern "C" {
 typedef enum ffs_status {
   FFS_OK = 0,
   FFS_ERR_INVALID_ARG = 1,   /* bad shape, size, weights, T, pointer ... */
-  FFS_ERR_UNSUPPORTED = 2,   /* valid request this build does not implement (e.g. tile too small) */
+  FFS_ERR_UNSUPPORTED = 2,   /* valid request the library does not implement (e.g. tile too small) */
   FFS_ERR_OUT_OF_MEMORY = 3, /* device or host allocation failed */
   FFS_ERR_HIP = 4,           /* a HIP runtime call failed */
   FFS_ERR_NO_DEVICE = 5,     /* no gfx950 device visible */

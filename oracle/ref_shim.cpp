@@ -14,7 +14,7 @@ int ref_stencil1d(const double* in, int n, const double* kernel, int k, double* 
   stencil1D(in, n, kernel, k, out);
   return 0;
 }
-// One step of the driver's dormant check composition (src/1D/1d_main.cu:220-223): reverse the
+// One step of the reference 1D main program's dormant check composition (src/1D/1d_main.cu:220-223): reverse the
 // kernel, run stencil1D, rotate the result right by k/2.
 int ref_step1d_driver(const double* in, int n, const double* kernel, int k, double* out) {
   std::vector<double> rk(kernel, kernel + k);
@@ -26,7 +26,7 @@ int ref_step1d_driver(const double* in, int n, const double* kernel, int k, doub
   return 0;
 }
 #elif defined(FFS_REF_2D)
-// One step of the driver's dormant check composition (src/2D/2d_main.cu:190-192):
+// One step of the reference 2D main program's dormant check composition (src/2D/2d_main.cu:190-192):
 // stencil2D(convertTo2D(in), rotateKernel180(kernel)) on a square width x width grid.
 int ref_step2d_driver(const double* in, int width, const double* kernel, int k, double* out) {
   std::vector<double> kv(kernel, kernel + k * k);
