@@ -371,6 +371,15 @@ bool out16_block(const ffs_plan* pl) {
   return pl->kind == ffsk::KIND_2D32 && pl->V[0] <= 16 && pl->V[1] <= 16;
 }
 
+// Wave-to-tile mapping of the FFT tile kernels: true = hardware order.  AUTO takes XCD-contiguous bands
+// for the FP64 32-point kernel (measured 4% faster on Box2D25P 8192^2: 1073 vs 1116 us per block) and
+// hardware order for the others (1-3% faster for the mixed forms).
+bool linear_map(const ffs_plan* pl) {
+  const ffs_problem& p = pl->prob;
+  if (p.mapping != FFS_MAP_AUTO) return p.mapping == FFS_MAP_LINEAR;
+  return !(pl->kind == ffsk::KIND_2D32 && p.precision == FFS_PREC_FP64);
+}
+
 ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, bool tail) {
   ffsk::TileArgs a{};
   a.src = src;
@@ -402,7 +411,7 @@ ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, boo
   a.h2 = h[2];
   a.precision = p.precision;
   a.scale_exp = tail ? pl->scale_tail : pl->scale_main;
-  a.linear_map = p.mapping == FFS_MAP_XCD ? 0 : 1;  // AUTO = hardware order (measured 1-3% faster)
+  a.linear_map = linear_map(pl) ? 1 : 0;
   a.out16 = out16_block(pl) ? 1 : 0;
   if (a.out16) {
     const size_t per = pl->prob.precision == FFS_PREC_FP64 ? sizeof(double) * ffsk::kInv32F64
@@ -737,7 +746,7 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   info->precision = p.precision;
   info->convention = p.convention;
   info->algorithm = pl->direct ? FFS_ALGO_DIRECT : FFS_ALGO_FFT;
-  info->mapping = p.mapping == FFS_MAP_XCD ? FFS_MAP_XCD : FFS_MAP_LINEAR;
+  info->mapping = linear_map(pl) ? FFS_MAP_LINEAR : FFS_MAP_XCD;
   if (pl->direct) {
     const ffsk::DirectArgs a = make_direct_args(pl, nullptr, nullptr, pl->t);
     const int r = p.radius, K = 2 * r + 1;

@@ -83,8 +83,9 @@ typedef enum ffs_algorithm { FFS_ALGO_FFT = 0, FFS_ALGO_DIRECT = 1, FFS_ALGO_AUT
 /* Wave-to-tile mapping of the FFT tile kernels: which workgroup takes which tile pairs.
  *   XCD:         workgroups are renumbered so that each of the 8 XCDs takes one contiguous band of tile
  *                pairs (neighbouring windows share that XCD's L2);
- *   AUTO/LINEAR: hardware dispatch order (consecutive workgroups go to different XCDs); measured 1-3%
- *                faster than XCD bands for the 2D forms on Box2D25P 8192^2.
+ *   LINEAR:      hardware dispatch order (consecutive workgroups go to different XCDs);
+ *   AUTO:        XCD for the 2D FP64 form on 32-point tiles (measured 4% faster on Box2D25P 8192^2),
+ *                LINEAR otherwise (1-3% faster than XCD bands for the mixed 2D forms there).
  * The direct kernels ignore it.  No reference counterpart (the reference launches one warp per tile
  * pair in grid order). */
 typedef enum ffs_mapping { FFS_MAP_AUTO = 0, FFS_MAP_XCD = 1, FFS_MAP_LINEAR = 2 } ffs_mapping;

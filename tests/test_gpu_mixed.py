@@ -208,3 +208,5 @@ def test_planner_depth_8192(ffs, torch_cuda, form, shape, depth):
     info = plan.info
     plan.close()
     assert info["tile"][:2] == [32, 32] and info["fuse"] == depth and info["valid"][:2] == [16, 16], info
+    # AUTO mapping: XCD bands for the FP64 32-point kernel, hardware order for the mixed forms
+    assert info["mapping"] == (1 if form == "fp64" else 2), info
