@@ -61,8 +61,12 @@ class Problem(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("algorithm", ctypes.c_int32),
         ("mapping", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("flags", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
+
+
+FLAG_FULL_STAGES = 0x1
 
 
 class PlanInfo(ctypes.Structure):
@@ -132,6 +136,8 @@ def load_library() -> ctypes.CDLL:
         "ffs_format_csv": [ctypes.c_char_p, ctypes.c_int32, P(ctypes.c_int64), ctypes.c_int64, ctypes.c_double,
                            ctypes.c_char_p, ctypes.c_int32],
         "ffs_autotune": [P(Problem), ctypes.c_char_p, P(ctypes.c_double)],
+        "ffs_plan_read_table": [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
+                                P(ctypes.c_int64)],
         "ffs_batch_create": [P(Problem), ctypes.c_int32, P(ctypes.c_void_p)],
         "ffs_batch_run": [ctypes.c_void_p, P(ctypes.c_void_p), P(ctypes.c_void_p), ctypes.c_void_p],
         "ffs_batch_get_info": [ctypes.c_void_p, ctypes.c_int32, P(PlanInfo)],
@@ -194,14 +200,45 @@ def format_csv(tag: str, sizes: Sequence[int], steps: int, ms: float) -> str:
     return buf.value.decode()
 
 
-def _ptr(x) -> int:
+def _ptr(x, numel: Optional[int] = None, device: Optional[int] = None) -> int:
+    """Device address of x.  For a torch tensor, also require `numel` elements and the plan's `device`
+    (a raw integer pointer cannot be checked: the caller vouches for its size)."""
     if isinstance(x, int):
         return x
     if hasattr(x, "data_ptr"):  # torch tensor
         if not x.is_cuda or x.dtype.__str__() != "torch.float64" or not x.is_contiguous():
             raise FFSError(1, "device buffers must be contiguous float64 CUDA tensors")
+        if numel is not None and x.numel() != numel:
+            raise FFSError(1, f"tensor has {x.numel()} elements, the grid has {numel}")
+        if device is not None and x.device.index != device:
+            raise FFSError(1, f"tensor is on cuda:{x.device.index}, the plan on device {device}")
         return x.data_ptr()
     raise TypeError(f"expected a device pointer or torch tensor, got {type(x)!r}")
+
+
+def _problem(dims: int, sizes: Sequence[int], w: np.ndarray, radius: int, steps: int, precision="fp64",
+             convention="ref", tile: int = 0, fuse: int = 0, use_graph: bool = True, device: int = 0,
+             algorithm="fft", mapping="auto", full_stages: bool = False) -> Problem:
+    """An ffs_problem for the given arguments (w must stay alive while the struct is used)."""
+    p = Problem()
+    _check(load_library().ffs_problem_init(ctypes.byref(p)))
+    p.dims = dims
+    p.radius = radius
+    sz = [int(s) for s in sizes] + [1] * (3 - len(sizes))
+    for i in range(3):
+        p.sizes[i] = sz[i]
+    p.weights = _dptr(w)
+    p.steps = int(steps)
+    p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    p.convention = CONVENTIONS[convention] if isinstance(convention, str) else int(convention)
+    p.tile = tile
+    p.fuse = fuse
+    p.use_graph = 1 if use_graph else 0
+    p.device = device
+    p.algorithm = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+    p.mapping = MAPPINGS[mapping] if isinstance(mapping, str) else int(mapping)
+    p.flags = FLAG_FULL_STAGES if full_stages else 0
+    return p
 
 
 class Plan:
@@ -209,29 +246,18 @@ class Plan:
 
     def __init__(self, dims: int, sizes: Sequence[int], weights, radius: int, steps: int,
                  precision: str = "fp64", convention: str = "ref", tile: int = 0, fuse: int = 0,
-                 use_graph: bool = True, device: int = 0, algorithm: str = "fft", mapping: str = "auto"):
+                 use_graph: bool = True, device: int = 0, algorithm: str = "fft", mapping: str = "auto",
+                 full_stages: bool = False):
         lib = load_library()
         self._w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).ravel())
-        p = Problem()
-        _check(lib.ffs_problem_init(ctypes.byref(p)))
-        p.dims = dims
-        p.radius = radius
-        sz = [int(s) for s in sizes] + [1] * (3 - len(sizes))
-        for i in range(3):
-            p.sizes[i] = sz[i]
-        p.weights = _dptr(self._w)
-        p.steps = int(steps)
-        p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        p.convention = CONVENTIONS[convention] if isinstance(convention, str) else int(convention)
-        p.tile = tile
-        p.fuse = fuse
-        p.use_graph = 1 if use_graph else 0
-        p.device = device
-        p.algorithm = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
-        p.mapping = MAPPINGS[mapping] if isinstance(mapping, str) else int(mapping)
+        p = _problem(dims, sizes, self._w, radius, steps, precision, convention, tile, fuse, use_graph, device,
+                     algorithm, mapping, full_stages)
+        sz = [int(s) for s in sizes]
         self.problem = p
         self.sizes = tuple(sz[:dims])
         self.dims = dims
+        self.device = device
+        self.numel = int(np.prod(self.sizes))
         h = ctypes.c_void_p()
         _check(lib.ffs_plan_create(ctypes.byref(p), ctypes.byref(h)))
         self._h = h
@@ -242,14 +268,25 @@ class Plan:
         _check(load_library().ffs_plan_get_info(self._h, ctypes.byref(inf)))
         return inf.as_dict()
 
+    def read_table(self, which: int) -> bytes:
+        """Raw bytes of one of the plan's device tables (0 ops, 1 mixed ops, 2 spectrum, 3 out16 inverse)."""
+        lib = load_library()
+        n = ctypes.c_int64()
+        _check(lib.ffs_plan_read_table(self._h, which, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib.ffs_plan_read_table(self._h, which, buf, n.value, ctypes.byref(n)))
+        return buf.raw
+
     def run(self, d_in, d_out, stream=None) -> None:
         """d_out = S^T(d_in); pointers or float64 CUDA tensors.  stream: int handle, torch stream or None
         (None = torch's current stream when torch is loaded, else the null stream)."""
-        _check(load_library().ffs_run(self._h, ctypes.c_void_p(_ptr(d_in)), ctypes.c_void_p(_ptr(d_out)),
+        pi, po = _ptr(d_in, self.numel, self.device), _ptr(d_out, self.numel, self.device)
+        _check(load_library().ffs_run(self._h, ctypes.c_void_p(pi), ctypes.c_void_p(po),
                                       ctypes.c_void_p(_stream_handle(stream))))
 
     def run_block(self, d_in, d_out, tail: bool = False, stream=None) -> None:
-        _check(load_library().ffs_run_block(self._h, ctypes.c_void_p(_ptr(d_in)), ctypes.c_void_p(_ptr(d_out)),
+        pi, po = _ptr(d_in, self.numel, self.device), _ptr(d_out, self.numel, self.device)
+        _check(load_library().ffs_run_block(self._h, ctypes.c_void_p(pi), ctypes.c_void_p(po),
                                             1 if tail else 0, ctypes.c_void_p(_stream_handle(stream))))
 
     def run_host(self, x: np.ndarray) -> np.ndarray:
@@ -340,6 +377,7 @@ class Batch:
         _check(lib.ffs_batch_create(arr, n, ctypes.byref(h)))
         self._h = h
         self.count = n
+        self.device = 0
 
     def info(self, i: int) -> dict:
         inf = PlanInfo()
@@ -348,8 +386,10 @@ class Batch:
 
     def run(self, d_ins, d_outs, stream=None) -> None:
         n = self.count
-        ins = (ctypes.c_void_p * n)(*[_ptr(x) for x in d_ins])
-        outs = (ctypes.c_void_p * n)(*[_ptr(x) for x in d_outs])
+        if len(d_ins) != n or len(d_outs) != n:
+            raise FFSError(1, f"batch of {n} grids got {len(d_ins)} inputs and {len(d_outs)} outputs")
+        ins = (ctypes.c_void_p * n)(*[_ptr(x, s[0] * s[1], self.device) for x, s in zip(d_ins, self.sizes)])
+        outs = (ctypes.c_void_p * n)(*[_ptr(x, s[0] * s[1], self.device) for x, s in zip(d_outs, self.sizes)])
         _check(load_library().ffs_batch_run(self._h, ins, outs, ctypes.c_void_p(_stream_handle(stream))))
 
     def close(self) -> None:

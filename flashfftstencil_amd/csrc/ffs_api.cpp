@@ -87,10 +87,17 @@ struct ffs_plan {
   int64_t npoints = 0;
   int64_t hlen = 0;
   hipStream_t cap_stream = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-  const double* g_in = nullptr;
-  double* g_out = nullptr;
+  // Captured block chains for the last few (d_in, d_out) pairs, least recently used replaced first.
+  struct GraphSlot {
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    const double* in = nullptr;
+    double* out = nullptr;
+    uint64_t used = 0;
+  };
+  static constexpr int kGraphSlots = 4;
+  GraphSlot graphs[kGraphSlots];
+  uint64_t graph_clock = 0;
 };
 
 struct ffs_batch {
@@ -121,6 +128,10 @@ ffs_status validate(const ffs_problem* p, int64_t* npoints) {
   if (!p->weights) return fail(FFS_ERR_INVALID_ARG, "weights pointer is NULL");
   if (p->precision < FFS_PREC_FP64 || p->precision > FFS_PREC_MIXED_F16)
     return fail(FFS_ERR_INVALID_ARG, "unknown precision %d", p->precision);
+  if (p->precision == FFS_PREC_MIXED_BF16X2)
+    return fail(FFS_ERR_UNSUPPORTED,
+                "the 2-term BF16 form is withdrawn (it misses 1e-6 before the field has smoothed); use "
+                "FFS_PREC_MIXED_F16 or FFS_PREC_MIXED_BF16");
   if (p->convention != FFS_CONV_REF && p->convention != FFS_CONV_CENTRED)
     return fail(FFS_ERR_INVALID_ARG, "unknown convention %d", p->convention);
   if (p->fuse < 0) return fail(FFS_ERR_INVALID_ARG, "fuse must be >= 0 (got %d)", p->fuse);
@@ -128,6 +139,7 @@ ffs_status validate(const ffs_problem* p, int64_t* npoints) {
     return fail(FFS_ERR_INVALID_ARG, "unknown algorithm %d", p->algorithm);
   if (p->mapping < FFS_MAP_AUTO || p->mapping > FFS_MAP_LINEAR)
     return fail(FFS_ERR_INVALID_ARG, "unknown mapping %d", p->mapping);
+  if (p->flags & ~FFS_FLAG_FULL_STAGES) return fail(FFS_ERR_INVALID_ARG, "unknown flags 0x%x", p->flags);
   int64_t n = 1;
   for (int d = 0; d < p->dims; d++) {
     if (p->sizes[d] < 1) return fail(FFS_ERR_INVALID_ARG, "size[%d] must be >= 1 (got %lld)", d, (long long)p->sizes[d]);
@@ -244,7 +256,7 @@ ffs_status choose_config(ffs_plan* pl) {
     pl->L[0] = 15;
     pl->L[1] = 16;
   } else if (p.dims == 2) {
-    const bool bf = p.precision == FFS_PREC_MIXED_BF16 || p.precision == FFS_PREC_MIXED_BF16X2;
+    const bool bf = p.precision == FFS_PREC_MIXED_BF16;
     const bool f16 = p.precision == FFS_PREC_MIXED_F16;
     const bool fp64 = p.precision == FFS_PREC_FP64;
     const int P = p.tile != 0 ? p.tile : (bf || f16 ? 32 : 16);
@@ -300,7 +312,7 @@ ffs_status choose_config(ffs_plan* pl) {
         const double tiles = std::ceil((double)p.sizes[d] / (double)V);
         pairs *= d == p.dims - 1 ? std::ceil(tiles / 2.0) : tiles;
       }
-      const bool out16 = kind == ffsk::KIND_2D32 && V <= 16;
+      const bool out16 = kind == ffsk::KIND_2D32 && V <= 16 && !(p.flags & FFS_FLAG_FULL_STAGES);
       const double c = compute * (p.dims == 3 ? 16.0 : 1.0) * (out16 ? cut : 1.0);
       const double cycles = (double)nb * (launch + pairs * std::max(c, hbm_per_point * Vd) / 1024.0);
       const double score = points * (double)T / cycles;
@@ -351,10 +363,15 @@ ffs_status choose_config(ffs_plan* pl) {
   return FFS_OK;
 }
 
+void drop_graph(ffs_plan::GraphSlot& g) {
+  if (g.gexec) (void)hipGraphExecDestroy(g.gexec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g = ffs_plan::GraphSlot{};
+}
+
 void release(ffs_plan* pl) {
   if (!pl) return;
-  if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
-  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  for (ffs_plan::GraphSlot& g : pl->graphs) drop_graph(g);
   if (pl->cap_stream) (void)hipStreamDestroy(pl->cap_stream);
   if (pl->d_ops) (void)hipFree(pl->d_ops);
   if (pl->d_opsm) (void)hipFree(pl->d_opsm);
@@ -368,7 +385,8 @@ void release(ffs_plan* pl) {
 // 32-point family: does the valid region fit one 16 x 16 block (reduced inverse stages)?  The main and
 // the tail block share the tiling, so both are out16 or neither; their inverse operands differ by shift.
 bool out16_block(const ffs_plan* pl) {
-  return pl->kind == ffsk::KIND_2D32 && pl->V[0] <= 16 && pl->V[1] <= 16;
+  return pl->kind == ffsk::KIND_2D32 && pl->V[0] <= 16 && pl->V[1] <= 16 &&
+         !(pl->prob.flags & FFS_FLAG_FULL_STAGES);
 }
 
 // Wave-to-tile mapping of the FFT tile kernels: true = hardware order.  AUTO takes XCD-contiguous bands
@@ -497,6 +515,137 @@ ffs_status enqueue_chain(ffs_plan* pl, const double* d_in, double* d_out, hipStr
   return FFS_OK;
 }
 
+// ---- plan set-up steps of ffs_plan_create ----
+
+// Validation, the weights and the planner's choice (host only, no HIP call).
+ffs_status plan_host(const ffs_problem* p, ffs_plan* pl) {
+  int64_t npoints = 0;
+  ffs_status st = validate(p, &npoints);
+  if (st != FFS_OK) return st;
+  pl->prob = *p;
+  const int K = 2 * p->radius + 1;
+  int64_t ntaps = 1;
+  for (int d = 0; d < p->dims; d++) ntaps *= K;
+  pl->weights.assign(p->weights, p->weights + ntaps);
+  pl->prob.weights = pl->weights.data();
+  pl->npoints = npoints;
+  return choose_config(pl);
+}
+
+// The plan's device exists and is a gfx950; makes it current and records its CU count (no allocation).
+ffs_status plan_device(ffs_plan* pl) {
+  int ndev = 0;
+  const hipError_t ce = hipGetDeviceCount(&ndev);
+  if (ce != hipSuccess || ndev <= 0)
+    return fail(FFS_ERR_NO_DEVICE, "no HIP device visible (%s, count %d)", hipGetErrorString(ce), ndev);
+  if (pl->prob.device < 0 || pl->prob.device >= ndev)
+    return fail(FFS_ERR_INVALID_ARG, "device %d out of range (%d devices)", pl->prob.device, ndev);
+  pl->dev = pl->prob.device;
+  hipDeviceProp_t prop;
+  if (hipSetDevice(pl->dev) != hipSuccess || hipGetDeviceProperties(&prop, pl->dev) != hipSuccess)
+    return fail(FFS_ERR_HIP, "cannot query device %d", pl->dev);
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(FFS_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950 only", pl->dev, prop.gcnArchName);
+  pl->num_cus = prop.multiProcessorCount;
+  return FFS_OK;
+}
+
+// Host constants: the constant-mode gains K(0)^t and, for the FP16 form, the tile scale exponents.
+ffs_status plan_constants(ffs_plan* pl) {
+  if (pl->direct) return FFS_OK;
+  const ffs_problem& p = pl->prob;
+  const int K = 2 * p.radius + 1;
+  pl->hlen = ffsk::spectrum_len(pl->kind);
+  {  // gain of the constant mode, K(0)^t = (sum w)^t, by repeated squaring in FP64
+    double sw = 0.0;
+    for (double wv : pl->weights) sw += wv;
+    auto ipow = [](double b, int64_t e) {
+      double r = 1.0;
+      for (; e > 0; e >>= 1, b *= b)
+        if (e & 1) r *= b;
+      return r;
+    };
+    pl->dc_main = ipow(sw, pl->t);
+    pl->dc_tail = ipow(sw, pl->t_tail > 0 ? pl->t_tail : pl->t);
+  }
+  if (p.precision == FFS_PREC_MIXED_F16) {
+    // largest |K(k)| over the 32 x 32 frequencies (direct DFT of the taps; |K| does not depend on the
+    // convention), then scale_exp = floor(10 - log2 max(1, |K|^t)) for each depth
+    double kmax = 0.0;
+    const int P = 32;
+    for (int k0 = 0; k0 < P; k0++)
+      for (int k1 = 0; k1 < P; k1++) {
+        double re = 0.0, im = 0.0;
+        for (int a0 = 0; a0 < K; a0++)
+          for (int a1 = 0; a1 < K; a1++) {
+            const double ang = -2.0 * M_PI * (double)((a0 * k0 + a1 * k1) % P) / P;
+            re += pl->weights[a0 * K + a1] * std::cos(ang);
+            im += pl->weights[a0 * K + a1] * std::sin(ang);
+          }
+        kmax = std::max(kmax, std::sqrt(re * re + im * im));
+      }
+    auto expo = [&](int64_t t) {
+      const double lg = kmax > 1.0 ? (double)t * std::log2(kmax) : 0.0;
+      return std::floor(10.0 - lg);
+    };
+    const double em = expo(pl->t), et = expo(pl->t_tail > 0 ? pl->t_tail : pl->t);
+    if (em < -100.0 || et < -100.0)
+      return fail(FFS_ERR_UNSUPPORTED, "FP16 form: |K|^t = 2^%.0f is beyond the FP16 range; use another form",
+                  10.0 - std::min(em, et));
+    pl->scale_main = (int32_t)em;
+    pl->scale_tail = (int32_t)et;
+  }
+  return FFS_OK;
+}
+
+// Bytes of each device table of a transform plan (0 = not used).
+struct TableSizes {
+  size_t ops, opsm, H, opsi, taps;
+};
+TableSizes table_sizes(const ffs_plan* pl) {
+  TableSizes t{};
+  if (pl->direct) return t;
+  t.ops = sizeof(double) * std::max(ffsk::kOpsLen, ffsk::kOps32F64);
+  t.opsm = pl->prob.precision != FFS_PREC_FP64 ? sizeof(uint32_t) * std::max(ffsk::kOpsMixedLen, ffsk::kOps32Words) : 0;
+  t.H = sizeof(double) * 4 * (size_t)ffsk::spectrum_len(pl->kind);
+  t.opsi = out16_block(pl) ? 2 * std::max(sizeof(double) * ffsk::kInv32F64, sizeof(uint32_t) * ffsk::kInv32Words) : 0;
+  t.taps = sizeof(double) * pl->weights.size();
+  return t;
+}
+
+// DFT operands and spectra into the plan's device tables, on stream s (the taps are already in d_taps).
+hipError_t plan_tables(ffs_plan* pl, hipStream_t s) {
+  const ffs_problem& p = pl->prob;
+  const int K = 2 * p.radius + 1;
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) {
+    if (x != hipSuccess && e == hipSuccess) e = x;
+  };
+  const int layout = p.precision == FFS_PREC_FP64 ? 0 : 1;
+  if (out16_block(pl)) {  // inverse operands of window columns h .. h + 15, main and tail depth
+    const int32_t hm = pl->h_main[1], ht = pl->t_tail > 0 ? pl->h_tail[1] : pl->h_main[1];
+    if (p.precision == FFS_PREC_FP64) {
+      double* o = static_cast<double*>(pl->d_opsi);
+      chk(ffsk::gen_inverse32_f64(o, hm, s));
+      chk(ffsk::gen_inverse32_f64(o + ffsk::kInv32F64, ht, s));
+    } else {
+      uint32_t* o = static_cast<uint32_t*>(pl->d_opsi);
+      chk(ffsk::gen_inverse32(o, p.precision == FFS_PREC_MIXED_F16, hm, s));
+      chk(ffsk::gen_inverse32(o + ffsk::kInv32Words, p.precision == FFS_PREC_MIXED_F16, ht, s));
+    }
+  }
+  if (pl->kind == ffsk::KIND_2D32 && p.precision == FFS_PREC_FP64)
+    chk(ffsk::gen_operands32_f64(pl->d_ops, s));
+  else if (pl->kind == ffsk::KIND_2D32)
+    chk(ffsk::gen_operands32(pl->d_opsm, p.precision == FFS_PREC_MIXED_F16, s));
+  else
+    chk(ffsk::gen_operands(pl->d_ops, pl->d_opsm, pl->L, s));
+  chk(ffsk::gen_spectrum(pl->d_H, pl->kind, layout, pl->L, pl->d_taps, K, p.dims, p.convention, p.radius, pl->t, s));
+  chk(ffsk::gen_spectrum(pl->d_H + 2 * pl->hlen, pl->kind, layout, pl->L, pl->d_taps, K, p.dims, p.convention,
+                         p.radius, pl->t_tail > 0 ? pl->t_tail : pl->t, s));
+  return e;
+}
+
 }  // namespace
 
 extern "C" {
@@ -592,140 +741,38 @@ ffs_status ffs_format_csv(const char* tag, int32_t dims, const int64_t* sizes, i
 ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
   if (!out) return fail(FFS_ERR_INVALID_ARG, "out is NULL");
   *out = nullptr;
-  int64_t npoints = 0;
-  ffs_status st = validate(p, &npoints);
-  if (st != FFS_OK) return st;
   ffs_plan* pl = new (std::nothrow) ffs_plan();
   if (!pl) return fail(FFS_ERR_OUT_OF_MEMORY, "host allocation failed");
-  pl->prob = *p;
-  const int K = 2 * p->radius + 1;
-  int64_t ntaps = 1;
-  for (int d = 0; d < p->dims; d++) ntaps *= K;
-  pl->weights.assign(p->weights, p->weights + ntaps);
-  pl->prob.weights = pl->weights.data();
-  pl->npoints = npoints;
-  st = choose_config(pl);
+  ffs_status st = plan_host(p, pl);
+  if (st == FFS_OK) st = plan_device(pl);
+  if (st == FFS_OK) st = plan_constants(pl);
   if (st != FFS_OK) {
     release(pl);
     return st;
   }
-  // ---- device work starts here ----
-  int ndev = 0;
-  const hipError_t ce = hipGetDeviceCount(&ndev);
-  if (ce != hipSuccess || ndev <= 0) {
-    release(pl);
-    return fail(FFS_ERR_NO_DEVICE, "no HIP device visible (%s, count %d)", hipGetErrorString(ce), ndev);
-  }
-  if (p->device < 0 || p->device >= ndev) {
-    release(pl);
-    return fail(FFS_ERR_INVALID_ARG, "device %d out of range (%d devices)", p->device, ndev);
-  }
-  pl->dev = p->device;
-  hipDeviceProp_t prop;
-  if (hipSetDevice(pl->dev) != hipSuccess || hipGetDeviceProperties(&prop, pl->dev) != hipSuccess) {
-    release(pl);
-    return fail(FFS_ERR_HIP, "cannot query device %d", pl->dev);
-  }
-  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
-    release(pl);
-    return fail(FFS_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950 only", pl->dev, prop.gcnArchName);
-  }
-  pl->num_cus = prop.multiProcessorCount;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) {
     if (x != hipSuccess && e == hipSuccess) e = x;
   };
-  if (pl->direct) {  // no transform tables: the taps travel in the kernel arguments
-    if (pl->nblocks >= 2) chk(hipMalloc(&pl->d_scratch, sizeof(double) * npoints));
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&pl->cap_stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-      release(pl);
-      return fail(e == hipErrorOutOfMemory ? FFS_ERR_OUT_OF_MEMORY : FFS_ERR_HIP, "plan set-up failed: %s",
-                  hipGetErrorString(e));
-    }
-    *out = pl;
-    return FFS_OK;
+  const TableSizes ts = table_sizes(pl);
+  if (!pl->direct) {
+    chk(hipMalloc(&pl->d_ops, ts.ops));
+    if (ts.opsm) chk(hipMalloc(&pl->d_opsm, ts.opsm));
+    chk(hipMalloc(&pl->d_H, ts.H));
+    if (ts.opsi) chk(hipMalloc(&pl->d_opsi, ts.opsi));
+    chk(hipMalloc(&pl->d_taps, ts.taps));
   }
-  pl->hlen = ffsk::spectrum_len(pl->kind);
-  chk(hipMalloc(&pl->d_ops, sizeof(double) * std::max(ffsk::kOpsLen, ffsk::kOps32F64)));
-  if (p->precision != FFS_PREC_FP64)
-    chk(hipMalloc(&pl->d_opsm, sizeof(uint32_t) * std::max(ffsk::kOpsMixedLen, ffsk::kOps32Words)));
-  chk(hipMalloc(&pl->d_H, sizeof(double) * 4 * pl->hlen));
-  if (out16_block(pl))
-    chk(hipMalloc(&pl->d_opsi, 2 * std::max(sizeof(double) * ffsk::kInv32F64, sizeof(uint32_t) * ffsk::kInv32Words)));
-  chk(hipMalloc(&pl->d_taps, sizeof(double) * ntaps));
-  if (pl->nblocks >= 2) chk(hipMalloc(&pl->d_scratch, sizeof(double) * npoints));
+  if (pl->nblocks >= 2) chk(hipMalloc(&pl->d_scratch, sizeof(double) * pl->npoints));
   if (e != hipSuccess) {
     release(pl);
     return fail(e == hipErrorOutOfMemory ? FFS_ERR_OUT_OF_MEMORY : FFS_ERR_HIP, "device allocation failed: %s",
                 hipGetErrorString(e));
   }
-  chk(hipMemcpy(pl->d_taps, pl->weights.data(), sizeof(double) * ntaps, hipMemcpyHostToDevice));
-  const int layout = p->precision == FFS_PREC_FP64 ? 0 : 1;
-  if (out16_block(pl)) {  // inverse operands of window columns h .. h + 15, main and tail depth
-    const int32_t hm = pl->h_main[1], ht = pl->t_tail > 0 ? pl->h_tail[1] : pl->h_main[1];
-    if (p->precision == FFS_PREC_FP64) {
-      double* o = static_cast<double*>(pl->d_opsi);
-      chk(ffsk::gen_inverse32_f64(o, hm, nullptr));
-      chk(ffsk::gen_inverse32_f64(o + ffsk::kInv32F64, ht, nullptr));
-    } else {
-      uint32_t* o = static_cast<uint32_t*>(pl->d_opsi);
-      chk(ffsk::gen_inverse32(o, p->precision == FFS_PREC_MIXED_F16, hm, nullptr));
-      chk(ffsk::gen_inverse32(o + ffsk::kInv32Words, p->precision == FFS_PREC_MIXED_F16, ht, nullptr));
-    }
+  if (!pl->direct) {
+    chk(hipMemcpy(pl->d_taps, pl->weights.data(), ts.taps, hipMemcpyHostToDevice));
+    chk(plan_tables(pl, nullptr));
+    chk(hipDeviceSynchronize());
   }
-  if (pl->kind == ffsk::KIND_2D32 && p->precision == FFS_PREC_FP64)
-    chk(ffsk::gen_operands32_f64(pl->d_ops, nullptr));
-  else if (pl->kind == ffsk::KIND_2D32)
-    chk(ffsk::gen_operands32(pl->d_opsm, p->precision == FFS_PREC_MIXED_F16, nullptr));
-  else
-    chk(ffsk::gen_operands(pl->d_ops, pl->d_opsm, pl->L, nullptr));
-  chk(ffsk::gen_spectrum(pl->d_H, pl->kind, layout, pl->L, pl->d_taps, K, p->dims, p->convention, p->radius, pl->t,
-                         nullptr));
-  chk(ffsk::gen_spectrum(pl->d_H + 2 * pl->hlen, pl->kind, layout, pl->L, pl->d_taps, K, p->dims, p->convention,
-                         p->radius, pl->t_tail > 0 ? pl->t_tail : pl->t, nullptr));
-  {  // gain of the constant mode, K(0)^t = (sum w)^t, by repeated squaring in FP64
-    double sw = 0.0;
-    for (double wv : pl->weights) sw += wv;
-    auto ipow = [](double b, int64_t e) {
-      double r = 1.0;
-      for (; e > 0; e >>= 1, b *= b)
-        if (e & 1) r *= b;
-      return r;
-    };
-    pl->dc_main = ipow(sw, pl->t);
-    pl->dc_tail = ipow(sw, pl->t_tail > 0 ? pl->t_tail : pl->t);
-  }
-  if (p->precision == FFS_PREC_MIXED_F16) {
-    // largest |K(k)| over the 32 x 32 frequencies (direct DFT of the taps; |K| does not depend on the
-    // convention), then scale_exp = floor(10 - log2 max(1, |K|^t)) for each depth
-    double kmax = 0.0;
-    const int P = 32;
-    for (int k0 = 0; k0 < P; k0++)
-      for (int k1 = 0; k1 < P; k1++) {
-        double re = 0.0, im = 0.0;
-        for (int a0 = 0; a0 < K; a0++)
-          for (int a1 = 0; a1 < K; a1++) {
-            const double ang = -2.0 * M_PI * (double)((a0 * k0 + a1 * k1) % P) / P;
-            re += pl->weights[a0 * K + a1] * std::cos(ang);
-            im += pl->weights[a0 * K + a1] * std::sin(ang);
-          }
-        kmax = std::max(kmax, std::sqrt(re * re + im * im));
-      }
-    auto expo = [&](int64_t t) {
-      const double lg = kmax > 1.0 ? (double)t * std::log2(kmax) : 0.0;
-      return std::floor(10.0 - lg);
-    };
-    const double em = expo(pl->t), et = expo(pl->t_tail > 0 ? pl->t_tail : pl->t);
-    if (em < -100.0 || et < -100.0) {
-      release(pl);
-      return fail(FFS_ERR_UNSUPPORTED, "FP16 form: |K|^t = 2^%.0f is beyond the FP16 range; use another form",
-                  10.0 - std::min(em, et));
-    }
-    pl->scale_main = (int32_t)em;
-    pl->scale_tail = (int32_t)et;
-  }
-  chk(hipDeviceSynchronize());
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&pl->cap_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     release(pl);
@@ -812,7 +859,6 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   const int dd = p.dims == 1 ? 2 : p.dims;
   double per_cplx = 3.0 * gemm;
   if (p.precision == FFS_PREC_MIXED_BF16) per_cplx = 4.0 * 3.0 * 2.0 * 16 * 16 * 32;
-  if (p.precision == FFS_PREC_MIXED_BF16X2) per_cplx = 4.0 * 2.0 * 2.0 * 16 * 16 * 32;
   if (p.precision == FFS_PREC_MIXED_F32) per_cplx = 4.0 * gemm;
   double per_pair = 2.0 * dd * per_cplx * std::pow(16.0, dd - 2);
   // 32-point family: output blocks over the 4 stages (4 + 4 + 4 + 4, or 4 + 4 + 2 + 1 for an out16 main
@@ -825,6 +871,29 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
                2.0 * 16 * 16 * 32;
   info->flops_per_block = per_pair * (double)pl->pairs;
   info->bytes_per_block = 16.0 * (double)pl->npoints;
+  return FFS_OK;
+}
+
+ffs_status ffs_plan_read_table(const ffs_plan* pl, int32_t which, void* host_out, int64_t capacity,
+                               int64_t* bytes) {
+  if (!pl) return fail(FFS_ERR_INVALID_ARG, "plan is NULL");
+  if (pl->direct) return fail(FFS_ERR_UNSUPPORTED, "the direct algorithm has no transform tables");
+  const TableSizes ts = table_sizes(pl);
+  const void* src = nullptr;
+  size_t n = 0;
+  switch (which) {
+    case FFS_TABLE_OPS: src = pl->d_ops; n = ts.ops; break;
+    case FFS_TABLE_OPS_MIXED: src = pl->d_opsm; n = ts.opsm; break;
+    case FFS_TABLE_SPECTRUM: src = pl->d_H; n = ts.H; break;
+    case FFS_TABLE_INVERSE16: src = pl->d_opsi; n = ts.opsi; break;
+    default: return fail(FFS_ERR_INVALID_ARG, "unknown table %d", which);
+  }
+  if (bytes) *bytes = (int64_t)n;
+  if (!src || n == 0) return fail(FFS_ERR_UNSUPPORTED, "this plan has no table %d", which);
+  if (!host_out) return FFS_OK;  // size query
+  if (capacity < (int64_t)n) return fail(FFS_ERR_INVALID_ARG, "capacity %lld < %zu bytes", (long long)capacity, n);
+  FFS_HIP(hipSetDevice(pl->dev));
+  FFS_HIP(hipMemcpy(host_out, src, n, hipMemcpyDeviceToHost));
   return FFS_OK;
 }
 
@@ -848,15 +917,14 @@ ffs_status ffs_run(ffs_plan* pl, const double* d_in, double* d_out, void* stream
   // Chains of a few launches go out as plain launches: replaying a graph costs ~9 us more per run on
   // MI355X than launching up to four kernels directly (measured on 1 to 4 block chains).
   if (!pl->prob.use_graph || pl->nblocks <= kGraphMinLaunches) return enqueue_chain(pl, d_in, d_out, s);
-  if (!pl->gexec || pl->g_in != d_in || pl->g_out != d_out) {
-    if (pl->gexec) {
-      FFS_HIP(hipGraphExecDestroy(pl->gexec));
-      pl->gexec = nullptr;
-    }
-    if (pl->graph) {
-      FFS_HIP(hipGraphDestroy(pl->graph));
-      pl->graph = nullptr;
-    }
+  ffs_plan::GraphSlot* slot = nullptr;
+  for (ffs_plan::GraphSlot& g : pl->graphs)
+    if (g.gexec && g.in == d_in && g.out == d_out) slot = &g;
+  if (!slot) {  // capture this pair into the least recently used slot
+    slot = &pl->graphs[0];
+    for (ffs_plan::GraphSlot& g : pl->graphs)
+      if (g.used < slot->used) slot = &g;
+    drop_graph(*slot);
     FFS_HIP(hipStreamBeginCapture(pl->cap_stream, hipStreamCaptureModeThreadLocal));
     ffs_status st = enqueue_chain(pl, d_in, d_out, pl->cap_stream);
     hipGraph_t g = nullptr;
@@ -866,12 +934,17 @@ ffs_status ffs_run(ffs_plan* pl, const double* d_in, double* d_out, void* stream
       return st;
     }
     if (ee != hipSuccess) return fail(FFS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ee));
-    pl->graph = g;
-    FFS_HIP(hipGraphInstantiate(&pl->gexec, pl->graph, nullptr, nullptr, 0));
-    pl->g_in = d_in;
-    pl->g_out = d_out;
+    slot->graph = g;
+    const hipError_t ie = hipGraphInstantiate(&slot->gexec, slot->graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      drop_graph(*slot);
+      return fail(FFS_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+    }
+    slot->in = d_in;
+    slot->out = d_out;
   }
-  FFS_HIP(hipGraphLaunch(pl->gexec, s));
+  slot->used = ++pl->graph_clock;
+  FFS_HIP(hipGraphLaunch(slot->gexec, s));
   return FFS_OK;
 }
 
@@ -908,15 +981,9 @@ ffs_status ffs_run_host(ffs_plan* pl, const double* h_in, double* h_out) {
   }
   (void)hipFree(din);
   (void)hipFree(dout);
-  // the cached graph referenced the temporaries: drop it
-  if (pl->gexec && pl->g_in == din) {
-    (void)hipGraphExecDestroy(pl->gexec);
-    (void)hipGraphDestroy(pl->graph);
-    pl->gexec = nullptr;
-    pl->graph = nullptr;
-    pl->g_in = nullptr;
-    pl->g_out = nullptr;
-  }
+  // a cached graph referenced the temporaries: drop it
+  for (ffs_plan::GraphSlot& g : pl->graphs)
+    if (g.gexec && g.in == din) drop_graph(g);
   if (st != FFS_OK) return st;
   if (e != hipSuccess) return fail(FFS_ERR_HIP, "run_host failed: %s", hipGetErrorString(e));
   return FFS_OK;
@@ -926,10 +993,45 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
   int64_t npoints = 0;
   ffs_status st = validate(p, &npoints);
   if (st != FFS_OK) return st;
+  // The key holds everything the admissible configurations and their timings depend on: the problem's
+  // shape, the weights themselves (star support bounds the direct depth, |K| the FP16 scale; FNV-1a over
+  // their bytes), the flags and the device architecture.
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(FFS_ERR_NO_DEVICE, "no HIP device visible");
+  if (p->device < 0 || p->device >= ndev) return fail(FFS_ERR_INVALID_ARG, "device %d out of range", p->device);
+  hipDeviceProp_t prop;
+  FFS_HIP(hipGetDeviceProperties(&prop, p->device));
+  char arch[32];
+  std::snprintf(arch, sizeof arch, "%.*s", 31, prop.gcnArchName);
+  for (char* c = arch; *c; c++)
+    if (*c == ':' || *c == '"' || *c == ' ') *c = '.';
+  int64_t ntaps_key = 1;
+  for (int d = 0; d < p->dims; d++) ntaps_key *= 2 * p->radius + 1;
+  uint64_t wh = 0xcbf29ce484222325ULL;
+  for (int64_t i = 0; i < ntaps_key; i++) {
+    uint64_t bits;
+    std::memcpy(&bits, &p->weights[i], sizeof bits);
+    for (int k = 0; k < 8; k++, bits >>= 8) wh = (wh ^ (bits & 0xff)) * 0x100000001b3ULL;
+  }
   char key[256];
-  std::snprintf(key, sizeof key, "d%d_r%d_p%d_c%d_a%d_n%lldx%lldx%lld_T%lld", p->dims, p->radius, p->precision,
-                p->convention, p->algorithm, (long long)p->sizes[0], (long long)(p->dims > 1 ? p->sizes[1] : 1),
-                (long long)(p->dims > 2 ? p->sizes[2] : 1), (long long)p->steps);
+  std::snprintf(key, sizeof key, "d%d_r%d_p%d_c%d_a%d_n%lldx%lldx%lld_T%lld_w%016llx_f%d_%s", p->dims, p->radius,
+                p->precision, p->convention, p->algorithm, (long long)p->sizes[0],
+                (long long)(p->dims > 1 ? p->sizes[1] : 1), (long long)(p->dims > 2 ? p->sizes[2] : 1),
+                (long long)p->steps, (unsigned long long)wh, p->flags, arch);
+  // host-only admissibility check of a configuration (the planner's own validation, no device work)
+  auto admissible = [&](int algo, int tile, int fuse, int mapping) {
+    ffs_plan probe;
+    probe.prob = *p;
+    probe.prob.algorithm = algo;
+    if (algo == FFS_ALGO_DIRECT) probe.prob.precision = FFS_PREC_FP64;
+    probe.prob.tile = algo == FFS_ALGO_DIRECT ? 0 : tile;
+    probe.prob.fuse = fuse;
+    probe.prob.mapping = mapping;
+    if (validate(&probe.prob, &probe.npoints) != FFS_OK) return false;
+    probe.weights.assign(p->weights, p->weights + ntaps_key);
+    probe.prob.weights = probe.weights.data();
+    return choose_config(&probe) == FFS_OK;
+  };
   auto apply = [&](int algo, int tile, int fuse, int mapping) {
     p->algorithm = algo;
     if (algo == FFS_ALGO_DIRECT) p->precision = FFS_PREC_FP64;
@@ -958,7 +1060,7 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
         }
       }
       std::fclose(f);
-      if (algo >= 0) {
+      if (algo >= 0 && admissible(algo, tile, fuse, mapping)) {  // else: stale or foreign line, tune again
         apply(algo, tile, fuse, mapping);
         if (best_ms) *best_ms = ms;
         return FFS_OK;
@@ -975,7 +1077,7 @@ ffs_status ffs_autotune(ffs_problem* p, const char* cache_path, double* best_ms)
     std::vector<int> tiles;
     if (p->dims == 1) tiles = {240};
     else if (p->dims == 3) tiles = {16};
-    else if (p->precision == FFS_PREC_MIXED_BF16 || p->precision == FFS_PREC_MIXED_BF16X2) tiles = {16, 32};
+    else if (p->precision == FFS_PREC_MIXED_BF16) tiles = {16, 32};
     else if (p->precision == FFS_PREC_MIXED_F16) tiles = {32};
     else if (p->precision == FFS_PREC_FP64) tiles = {16, 32};
     else tiles = {16};
@@ -1075,8 +1177,7 @@ ffs_status ffs_batch_create(const ffs_problem* problems, int32_t count, ffs_batc
       return fail(FFS_ERR_UNSUPPORTED, "batch problem %d: batches run the FFT algorithm only", i);
     if (problems[i].precision != prec)
       return fail(FFS_ERR_INVALID_ARG, "batch problem %d: all problems must share one precision", i);
-    if (prec != FFS_PREC_FP64 && prec != FFS_PREC_MIXED_BF16 && prec != FFS_PREC_MIXED_BF16X2 &&
-        prec != FFS_PREC_MIXED_F16)
+    if (prec != FFS_PREC_FP64 && prec != FFS_PREC_MIXED_BF16 && prec != FFS_PREC_MIXED_F16)
       return fail(FFS_ERR_UNSUPPORTED, "batches run the FP64, BF16 and FP16 forms (32-point tiles), not FP32");
     if (problems[i].tile != 0 && problems[i].tile != 32)
       return fail(FFS_ERR_UNSUPPORTED, "batch problem %d: tile must be 32", i);

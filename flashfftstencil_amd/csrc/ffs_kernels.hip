@@ -570,17 +570,13 @@ typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 #define FFS_MFMA_BF16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16((a), (b), (c), 0, 0, 0)
 #define FFS_MFMA_F32(a, b, c) __builtin_amdgcn_mfma_f32_16x16x4f32((a), (b), (c), 0, 0, 0)
 
-// Input spacing of the 32-point stages.  With the inverse stages cut to the valid output block, the
-// compiler's schedule put the split of a stage's input 8-9 wait states after the MFMAs that produced it
-// (its own minimum for v_mfma_f32_16x16x32_{f16,bf16} on gfx950), and on MI355X that code returned
-// results that differ from the full stages by up to 1e-5 (FP16 form; 1e-7 expected).  With the four input
-// blocks of each split first passed through a 16-wait-state step, the reduced stages are bit-identical
-// to the full ones.  The step ties only the registers it names; it is not volatile.
-__device__ __forceinline__ void mfma_settle(f4& a, f4& b, f4& c, f4& d) {
-  asm("s_nop 7\n\ts_nop 7" : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
-}
+// No inline assembly anywhere in these kernels.  On gfx950 a VALU read of a VGPR that an MFMA wrote, and
+// an MFMA read (SrcA/B/C) of a VGPR that a VALU wrote, need software wait states (the hardware does not
+// interlock them); the compiler's hazard recognizer inserts them, but it does not see an inline-asm
+// statement as a VALU instruction, so an asm that consumes an MFMA result or produces an MFMA operand
+// gets no wait states.  Every split below is therefore plain code the compiler schedules itself.
 
-enum { PREC_BF16 = 1, PREC_F32 = 2, PREC_BF16X2 = 3, PREC_F16 = 4 };
+enum { PREC_BF16 = 1, PREC_F32 = 2, PREC_F16 = 4 };
 
 template <int NTERMS>
 struct OpsBF16 {
@@ -912,8 +908,7 @@ __device__ __forceinline__ void stage32(const Blk4& zr, const Blk4& zi, const u4
 #pragma unroll
   for (int I = 0; I < NI; I++) {
     bf8 rh, rm, rl, ih, im, il;
-    f4 x0 = zr.b[I], x1 = zr.b[2 + I], y0 = zi.b[I], y1 = zi.b[2 + I];
-    mfma_settle(x0, x1, y0, y1);
+    const f4 x0 = zr.b[I], x1 = zr.b[2 + I], y0 = zi.b[I], y1 = zi.b[2 + I];
     split8<NTERMS>(x0, x1, rh, rm, rl);
     split8<NTERMS>(y0, y1, ih, im, il);
 #pragma unroll
@@ -1086,15 +1081,11 @@ __device__ __forceinline__ uint32_t pk_f16(float a, float b) {
   return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{a, b}, h2));
 }
 
-// The FP16 residuals (a - h.lo, b - h.hi) of one word, each rounded once to FP16: v_fma_mixlo_f16 and
-// v_fma_mixhi_f16 widen the FP16 half, subtract it from the FP32 value and round the difference, in one
-// instruction per value (bit-identical to converting back, subtracting in FP32 and converting, which
-// takes five instructions per word; checked over 2^20 random pairs spanning 40 binades).
+// The FP16 residuals (a - h.lo, b - h.hi) of one word, each rounded once to FP16.  a - h is exact in FP32
+// (h is a's own FP16 rounding), so the only rounding is the final conversion.
 __device__ __forceinline__ uint32_t resid_f16(uint32_t hw, float a, float b) {
-  uint32_t lw;
-  asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(lw) : "v"(hw), "v"(a));
-  asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(lw) : "v"(hw), "v"(b));
-  return lw;
+  const h2 h = __builtin_bit_cast(h2, hw);
+  return pk_f16(a - (float)h[0], b - (float)h[1]);
 }
 
 // Same element order as split8: word k of the operand holds elements (2k, 2k+1) of (x0 | x1).
@@ -1111,22 +1102,20 @@ __device__ __forceinline__ void split8_f16(const f4& x0, const f4& x1, h8& h, h8
   l = as_h8(lw);
 }
 
-// split8_f16 of (x0 | x1) * sc for a power-of-two sc, with the scaling folded into the mixed FMAs:
-// h = f16(x sc) and l = f16(x sc - h), each one v_fma_mix with a single rounding.  x sc is exact in FP32
-// (a power-of-two scale of a normal value), so the words equal split8_f16 of the scaled values.
+// split8_f16 of (x0 | x1) * sc for a power-of-two sc: h = f16(x sc), l = f16(fma(x, sc, -h)).  x sc is
+// exact in FP32 (a power-of-two scale of a normal value) and so is x sc - h, so the words equal
+// split8_f16 of the scaled values; the compiler selects v_fma_mix for the residual FMA.
 __device__ __forceinline__ void split8_f16_scaled(const f4& x0, const f4& x1, float sc, h8& h, h8& l) {
   u4 hw, lw;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const f4& x = k < 2 ? x0 : x1;
     const int e = 2 * (k & 1);
-    uint32_t w, r;
-    asm("v_fma_mixlo_f16 %0, %1, %2, 0" : "=v"(w) : "v"(x[e]), "v"(sc));
-    asm("v_fma_mixhi_f16 %0, %1, %2, 0" : "+v"(w) : "v"(x[e + 1]), "v"(sc));
-    asm("v_fma_mixlo_f16 %0, %1, %2, -%3 op_sel_hi:[0,0,1]" : "=v"(r) : "v"(x[e]), "v"(sc), "v"(w));
-    asm("v_fma_mixhi_f16 %0, %1, %2, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(r) : "v"(x[e + 1]), "v"(sc), "v"(w));
-    hw[k] = w;
-    lw[k] = r;
+    const _Float16 h0 = (_Float16)(x[e] * sc), h1 = (_Float16)(x[e + 1] * sc);
+    const _Float16 l0 = (_Float16)__builtin_fmaf(x[e], sc, -(float)h0);
+    const _Float16 l1 = (_Float16)__builtin_fmaf(x[e + 1], sc, -(float)h1);
+    hw[k] = __builtin_bit_cast(uint32_t, h2{h0, h1});
+    lw[k] = __builtin_bit_cast(uint32_t, h2{l0, l1});
   }
   h = as_h8(hw);
   l = as_h8(lw);
@@ -1164,8 +1153,7 @@ __device__ __forceinline__ void stage32_f16(const Blk4& zr, const Blk4& zi, cons
 #pragma unroll
   for (int I = 0; I < NI; I++) {
     h8 rh, rl, ih, il;
-    f4 x0 = zr.b[I], x1 = zr.b[2 + I], y0 = zi.b[I], y1 = zi.b[2 + I];
-    mfma_settle(x0, x1, y0, y1);
+    const f4 x0 = zr.b[I], x1 = zr.b[2 + I], y0 = zi.b[I], y1 = zi.b[2 + I];
     if (SCALED) {
       split8_f16_scaled(x0, x1, sc, rh, rl);
       split8_f16_scaled(y0, y1, sc, ih, il);
@@ -1371,8 +1359,6 @@ hipError_t launch_batch(const TileArgs* d_items, const int64_t* d_prefix, int32_
     hipLaunchKernelGGL(k_batch32_f64, g, b, 0, s, d_items, d_prefix, n);
   else if (precision == PREC_BF16)
     hipLaunchKernelGGL(k_batch32_bf16<3>, g, b, 0, s, d_items, d_prefix, n);
-  else if (precision == PREC_BF16X2)
-    hipLaunchKernelGGL(k_batch32_bf16<2>, g, b, 0, s, d_items, d_prefix, n);
   else if (precision == PREC_F16)
     hipLaunchKernelGGL(k_batch32_f16, g, b, 0, s, d_items, d_prefix, n);
   else
@@ -1948,8 +1934,6 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
     const dim3 g3((unsigned)blocks3), b3(256);
     if (a.precision == PREC_BF16)
       hipLaunchKernelGGL((k_tile3d<PREC_BF16, 3>), g3, b3, 0, s, a);
-    else if (a.precision == PREC_BF16X2)
-      hipLaunchKernelGGL((k_tile3d<PREC_BF16, 2>), g3, b3, 0, s, a);
     else if (a.precision == PREC_F32)
       hipLaunchKernelGGL((k_tile3d<PREC_F32, 1>), g3, b3, 0, s, a);
     else
@@ -1968,8 +1952,6 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
   if (kind == KIND_2D16) {
     if (a.precision == PREC_BF16)
       hipLaunchKernelGGL((k_tile16_mixed<KIND_2D16, PREC_BF16, 3>), g, b, 0, s, a);
-    else if (a.precision == PREC_BF16X2)
-      hipLaunchKernelGGL((k_tile16_mixed<KIND_2D16, PREC_BF16, 2>), g, b, 0, s, a);
     else if (a.precision == PREC_F32)
       hipLaunchKernelGGL((k_tile16_mixed<KIND_2D16, PREC_F32, 1>), g, b, 0, s, a);
     else
@@ -1977,8 +1959,6 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
   } else if (kind == KIND_1DPFA) {
     if (a.precision == PREC_BF16)
       hipLaunchKernelGGL((k_tile16_mixed<KIND_1DPFA, PREC_BF16, 3>), g, b, 0, s, a);
-    else if (a.precision == PREC_BF16X2)
-      hipLaunchKernelGGL((k_tile16_mixed<KIND_1DPFA, PREC_BF16, 2>), g, b, 0, s, a);
     else if (a.precision == PREC_F32)
       hipLaunchKernelGGL((k_tile16_mixed<KIND_1DPFA, PREC_F32, 1>), g, b, 0, s, a);
     else
@@ -1988,10 +1968,6 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
       hipLaunchKernelGGL((k_tile32_bf16<3, true>), g, b, 0, s, a);
     else if (a.precision == PREC_BF16)
       hipLaunchKernelGGL((k_tile32_bf16<3, false>), g, b, 0, s, a);
-    else if (a.precision == PREC_BF16X2 && a.out16)
-      hipLaunchKernelGGL((k_tile32_bf16<2, true>), g, b, 0, s, a);
-    else if (a.precision == PREC_BF16X2)
-      hipLaunchKernelGGL((k_tile32_bf16<2, false>), g, b, 0, s, a);
     else if (a.precision == PREC_F16 && a.out16)
       hipLaunchKernelGGL(k_tile32_f16<true>, g, b, 0, s, a);
     else if (a.precision == PREC_F16)

@@ -14,6 +14,7 @@
 //   * the grid really advances T steps (the reference relaunches on the same input);
 //   * 1D reports points x T / time (the reference multiplies by the fused radius, 1d_main.cu:195);
 //   * `--custom FILE` (or `-` for stdin) reads (2r+1)^d weights, row-major, in the ref convention;
+//   * `--output FILE` writes the grid after the T steps (raw FP64, row-major, native byte order);
 //   * `--help` prints the help and exits 0.
 #include <chrono>
 #include <cstdio>
@@ -55,7 +56,9 @@ static void print_help() {
       "                       auto (direct where offered)\n"
       "  --mapping M          linear (default: hardware workgroup order) | xcd (each XCD takes a contiguous\n"
       "                       band of tiles)\n"
-      "  --seed S             seed of the weights and the initial grid (default 0x5EED0000)\n"
+      "  --seed S             seed of the weights and the initial grid (default 0x5EED0000; the grid is\n"
+      "                       ffs_fill_uniform(S + 1), the weights ffs_stencil_named(shape, S))\n"
+      "  --output FILE        write the grid after the T steps to FILE (raw FP64, row-major)\n"
       "  --fuse t             fused time depth (default automatic)\n"
       "  --no-graph           plain launches instead of a hipGraph replay\n"
       "  --tune [CACHE]       time the admissible tile/depth choices first (cached in CACHE, a JSON-lines file)\n",
@@ -81,7 +84,7 @@ static bool parse_i64(const char* s, long long* v) {
 
 int main(int argc, char** argv) {
   std::vector<std::string> pos;
-  std::string custom, prec = "fp64", conv = "ref", algo = "fft", mapping = "linear";
+  std::string custom, output, prec = "fp64", conv = "ref", algo = "fft", mapping = "linear";
   unsigned long long seed = 0x5EED0000ULL;
   int fuse = 0, use_graph = 1;
   bool tune = false;
@@ -101,6 +104,8 @@ int main(int argc, char** argv) {
       return 0;
     } else if (a == "--custom") {
       custom = need("--custom");
+    } else if (a == "--output") {
+      output = need("--output");
     } else if (a == "--precision") {
       prec = need("--precision");
     } else if (a == "--convention") {
@@ -229,6 +234,19 @@ int main(int argc, char** argv) {
   int64_t s64[3] = {sizes[0], sizes[1], sizes[2]};
   CHECK_FFS(ffs_format_csv(tag, dims, s64, T, ms, line, sizeof line));
   std::cout << line << std::endl;
+  if (!output.empty()) {
+    if (hipMemcpy(h.data(), dout, sizeof(double) * npts, hipMemcpyDeviceToHost) != hipSuccess) {
+      std::fprintf(stderr, "copy of the result failed\n");
+      return 2;
+    }
+    std::FILE* f = std::fopen(output.c_str(), "wb");
+    if (!f || std::fwrite(h.data(), sizeof(double), (size_t)npts, f) != (size_t)npts) {
+      std::fprintf(stderr, "cannot write %s\n", output.c_str());
+      if (f) std::fclose(f);
+      return 1;
+    }
+    std::fclose(f);
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipFree(din);

@@ -55,7 +55,8 @@ typedef enum ffs_precision {
   FFS_PREC_FP64 = 0,       /* FP64 MFMA (v_mfma_f64_16x16x4f64) contractions: the drop-in form */
   FFS_PREC_MIXED_BF16 = 1,  /* 3-term BF16 split operands, FP32 accumulation (v_mfma_f32_16x16x32_bf16) */
   FFS_PREC_MIXED_F32 = 2,   /* FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4f32) */
-  FFS_PREC_MIXED_BF16X2 = 3, /* 2-term BF16 data split (16 bits): faster, meets 1e-6 only on smoothed fields */
+  FFS_PREC_MIXED_BF16X2 = 3, /* withdrawn (FFS_ERR_UNSUPPORTED): a 2-term BF16 data split (16 bits) misses 1e-6
+                                before the field has smoothed, and was not faster than the 3-term form */
   FFS_PREC_MIXED_F16 = 4     /* 2-term FP16 split operands (22 bits) with a per-tile power-of-2 scale, FP32
                                 accumulation (v_mfma_f32_16x16x32_f16), 3 products per block; 2D only */
 } ffs_precision;
@@ -90,6 +91,13 @@ typedef enum ffs_algorithm { FFS_ALGO_FFT = 0, FFS_ALGO_DIRECT = 1, FFS_ALGO_AUT
  * pair in grid order). */
 typedef enum ffs_mapping { FFS_MAP_AUTO = 0, FFS_MAP_XCD = 1, FFS_MAP_LINEAR = 2 } ffs_mapping;
 
+/* ffs_problem.flags bits.
+ *   FULL_STAGES: on 32-point tiles whose valid region fits one 16 x 16 block (V <= 16), compute every
+ *                output block of the two inverse stages instead of only the kept one.  The results are
+ *                bitwise the same (the kept values come from the same operands in the same order); it
+ *                exists to check exactly that, and costs ~1.4x the tile time.  No reference counterpart. */
+#define FFS_FLAG_FULL_STAGES 0x1
+
 typedef struct ffs_problem {
   int32_t dims;          /* 1, 2 or 3 */
   int32_t radius;        /* r >= 0; taps per axis K = 2r+1 */
@@ -106,7 +114,8 @@ typedef struct ffs_problem {
   int32_t device;        /* HIP device ordinal */
   int32_t algorithm;     /* ffs_algorithm (default FFS_ALGO_FFT) */
   int32_t mapping;       /* ffs_mapping (default FFS_MAP_AUTO) */
-  int32_t reserved[2];
+  int32_t flags;         /* FFS_FLAG_* bits (default 0) */
+  int32_t reserved;
 } ffs_problem;
 
 /* What a plan chose. */
@@ -154,6 +163,24 @@ ffs_status ffs_fill_uniform_device(uint64_t seed, double* d_out, int64_t n, void
  * operands and the FP64 spectrum power on the GPU.  Validation happens before any HIP call. */
 ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out);
 ffs_status ffs_plan_get_info(const ffs_plan* plan, ffs_plan_info* info);
+
+/* Diagnostics: copy one of the plan's device tables, as generated on the GPU, to host memory (for unit
+ * checks of the DFT operands and the spectrum).  `bytes` receives the table's size; host_out = NULL only
+ * queries it.  Layouts are internal (per-lane MFMA operand order, see the library sources):
+ *   OPS:        FP64 DFT operands (16-point families: [axis][Fr, -(Fr+Fi), Fi-Fr][chunk][lane] doubles;
+ *               32-point FP64: [R][chunk][column block][Fr, -(Fr+Fi), Fi-Fr][lane])
+ *   OPS_MIXED:  split operands of the mixed forms (BF16/FP16 words, FP32 chunks)
+ *   SPECTRUM:   [main | tail] x [re plane | im plane] of K^t / prod(P), in the tile family's order
+ *   INVERSE16:  out16 inverse operands of the main and the tail depth
+ * No reference counterpart (the reference's tables live in __constant__ memory, src/2D/create_fft_pfa_plan.cu:25-107). */
+typedef enum ffs_table {
+  FFS_TABLE_OPS = 0,
+  FFS_TABLE_OPS_MIXED = 1,
+  FFS_TABLE_SPECTRUM = 2,
+  FFS_TABLE_INVERSE16 = 3
+} ffs_table;
+ffs_status ffs_plan_read_table(const ffs_plan* plan, int32_t which, void* host_out, int64_t capacity,
+                               int64_t* bytes);
 ffs_status ffs_plan_destroy(ffs_plan* plan);
 
 /* Advance T steps: d_out = S^T(d_in).  Device pointers to compact row-major FP64 grids of the plan's

@@ -40,7 +40,7 @@ def test_library_exports_every_declared_symbol(ffs):
 
 
 def test_struct_layouts_match_header(ffs):
-    # ffs_problem: 2*i32, 3*i64, ptr, i64, 8*i32 (algorithm, mapping last), 2*i32 reserved
+    # ffs_problem: 2*i32, 3*i64, ptr, i64, 8*i32 (algorithm, mapping last), flags, reserved
     assert ctypes.sizeof(ffs.Problem) == 4 * 2 + 8 * 3 + 8 + 8 + 4 * 8 + 4 * 2
     lib = ffs.load_library()
     p = ffs.Problem()
@@ -164,15 +164,64 @@ def test_c_struct_layout_matches_binding(ffs, tmp_path):
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "ffstencil.h"\n'
         'int main(void) {\n'
-        '  printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ffs_problem), offsetof(ffs_problem, algorithm),\n'
+        '  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %d\\n", sizeof(ffs_problem), offsetof(ffs_problem, algorithm),\n'
         '         offsetof(ffs_problem, weights), sizeof(ffs_plan_info), offsetof(ffs_plan_info, algorithm),\n'
         '         offsetof(ffs_plan_info, flops_per_block), offsetof(ffs_problem, mapping),\n'
-        '         offsetof(ffs_plan_info, mapping));\n'
+        '         offsetof(ffs_plan_info, mapping), offsetof(ffs_problem, flags), FFS_FLAG_FULL_STAGES);\n'
         '  return 0;\n}\n')
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(ffs.Problem), ffs.Problem.algorithm.offset, ffs.Problem.weights.offset,
             ctypes.sizeof(ffs.PlanInfo), ffs.PlanInfo.algorithm.offset, ffs.PlanInfo.flops_per_block.offset,
-            ffs.Problem.mapping.offset, ffs.PlanInfo.mapping.offset]
+            ffs.Problem.mapping.offset, ffs.PlanInfo.mapping.offset, ffs.Problem.flags.offset,
+            ffs.FLAG_FULL_STAGES]
     assert got == want
+
+
+def test_plan_create_rejects_unknown_flags(ffs):
+    st, msg = _status(ffs, sizes=[64, 64], flags=0x6)
+    assert st == 1 and "flags" in msg
+
+
+def test_bf16x2_withdrawn(ffs):
+    st, msg = _status(ffs, sizes=[64, 64], precision=3)
+    assert st == 2 and "withdrawn" in msg
+
+
+class _FakeTensor:
+    """Stands in for a CUDA float64 tensor: what the binding inspects before passing a pointer on."""
+
+    def __init__(self, n, device=0, ptr=0x1000):
+        self._n, self._ptr = n, ptr
+        self.is_cuda = True
+        self.dtype = "torch.float64"
+        self.device = type("dev", (), {"index": device})()
+
+    def is_contiguous(self):
+        return True
+
+    def numel(self):
+        return self._n
+
+    def data_ptr(self):
+        return self._ptr
+
+
+def test_binding_checks_tensor_size_and_device(ffs):
+    # an undersized tensor or one on another device would let the kernels read or write out of bounds
+    assert ffs._ptr(_FakeTensor(100), 100, 0) == 0x1000
+    with pytest.raises(ffs.FFSError, match="elements"):
+        ffs._ptr(_FakeTensor(99), 100, 0)
+    with pytest.raises(ffs.FFSError, match="cuda:1"):
+        ffs._ptr(_FakeTensor(100, device=1), 100, 0)
+    assert ffs._ptr(0x2000, 100, 0) == 0x2000  # raw pointers are the caller's responsibility
+
+
+def test_batch_run_checks_counts(ffs):
+    b = object.__new__(ffs.Batch)  # no device: only the argument checks of run() are exercised
+    b._h, b.count, b.device, b.sizes = None, 2, 0, [(4, 5), (6, 7)]
+    with pytest.raises(ffs.FFSError, match="2 grids"):
+        b.run([_FakeTensor(20)], [_FakeTensor(20)])
+    with pytest.raises(ffs.FFSError, match="elements"):
+        b.run([_FakeTensor(20), _FakeTensor(41)], [_FakeTensor(20), _FakeTensor(42)])

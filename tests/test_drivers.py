@@ -8,6 +8,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -99,8 +100,40 @@ def test_csv_line_on_gpu(args, tag, dims):
 
 @pytest.mark.gpu
 def test_custom_weights_on_gpu(tmp_path):
+    # w[1][1] = 1 in the ref convention (out[p] = sum_a w[a] in[p - a]) moves the grid by (1, 1) per step:
+    # after 7 steps the output is the initial grid rolled by (7, 7)
     f = tmp_path / "w.txt"
     f.write_text("0 0 0\n0 1 0\n0 0 0\n")
-    r = run("2d", "Box2D9P", "96", "7", "--custom", str(f))
+    out = tmp_path / "out.bin"
+    r = run("2d", "Box2D9P", "96", "7", "--custom", str(f), "--output", str(out))
     assert r.returncode == 0, r.stderr
     assert r.stdout.startswith("FlashFFTStencil, box_2d1r, 1, 96, 96, 7, ")
+    from tests import _oracle
+    x = _oracle.uniform(0x5EED0000 + 1, (96, 96))
+    got = np.fromfile(out, dtype=np.float64).reshape(96, 96)
+    assert np.abs(got - np.roll(x, (7, 7), axis=(0, 1))).max() < 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims,shape,size,steps,extra", [
+    ("2d", "Box2D25P", (70, 50), 9, ()), ("2d", "Box2D25P", (70, 50), 9, ("--precision", "f16")),
+    ("3d", "Box3D27P", (20, 18, 22), 4, ()), ("1d", "1D7P", (3000,), 12, ("--convention", "centred")),
+])
+def test_custom_weights_match_oracle(tmp_path, dims, shape, size, steps, extra):
+    # random (unnormalised, signed) --custom weights: the driver's grid equals the oracle's on the
+    # driver's own seeded input
+    from tests import _oracle
+    d = len(size)
+    radius = {"Box2D25P": 2, "Box3D27P": 1, "1D7P": 3}[shape]
+    w = np.random.default_rng(len(size) * 7 + steps).standard_normal((2 * radius + 1) ** d) * 0.2
+    f = tmp_path / "w.txt"
+    f.write_text(" ".join(repr(float(v)) for v in w))
+    out = tmp_path / "out.bin"
+    r = run(dims, shape, *map(str, size), str(steps), "--custom", str(f), "--output", str(out), *extra)
+    assert r.returncode == 0, r.stderr
+    x = _oracle.uniform(0x5EED0000 + 1, size)
+    conv = 1 if "centred" in extra else 0
+    ref = _oracle.sweep(x, radius, w, steps, convention=conv)
+    got = np.fromfile(out, dtype=np.float64).reshape(size)
+    tol = 1e-6 if "f16" in extra else 1e-12
+    assert _oracle.errors(got, ref)[1] <= tol

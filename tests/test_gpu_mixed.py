@@ -35,7 +35,7 @@ def test_2d_mixed_named(ffs, oracle, torch_cuda, form, shape, size, steps):
     assert rel <= TOL_MIXED, (form, shape, size, steps, info, rel)
 
 
-@pytest.mark.parametrize("form", ["bf16", "f32", "bf16x2", "f16"])
+@pytest.mark.parametrize("form", ["bf16", "f32", "f16"])
 def test_2d_mixed_long_horizon(ffs, oracle, torch_cuda, form):
     _, r, w, _ = ffs.named_stencil("Box2D25P", seed=0x5EED0002)
     x = oracle.uniform(0x5EED0003, (256, 256))
@@ -62,7 +62,7 @@ def test_2d_mixed_fuse_convention(ffs, oracle, torch_cuda, form, convention, fus
 # Valid widths on both sides of 16 in the 32-point family: V = 32 - 2 r t <= 16 takes the reduced inverse
 # stages (shifted spectrum, one output block), V > 16 the full ones.  Steps 2 t + 1 add a tail block of
 # depth 1 whose shift differs from the main block's.
-@pytest.mark.parametrize("form", ["bf16", "bf16x2", "f16"])
+@pytest.mark.parametrize("form", ["bf16", "f16"])
 @pytest.mark.parametrize("convention", ["ref", "centred"])
 @pytest.mark.parametrize("radius,fuse", [(1, 7), (1, 8), (1, 9), (2, 3), (2, 4), (2, 5), (3, 3), (3, 5)])
 def test_2d_mixed_valid_16_boundary(ffs, oracle, torch_cuda, form, convention, radius, fuse):
@@ -74,8 +74,7 @@ def test_2d_mixed_valid_16_boundary(ffs, oracle, torch_cuda, form, convention, r
     got, info = _run(ffs, torch_cuda, x, radius, w, steps, precision=form, convention=convention, fuse=fuse)
     assert info["tile"][:2] == [32, 32] and info["valid"][0] == 32 - 2 * radius * fuse
     ref = oracle.sweep(x, radius, w, steps, convention=0 if convention == "ref" else 1)
-    tol = 2e-5 if form == "bf16x2" else TOL_MIXED
-    assert oracle.errors(got, ref)[1] <= tol, (form, convention, radius, fuse, info)
+    assert oracle.errors(got, ref)[1] <= TOL_MIXED, (form, convention, radius, fuse, info)
 
 
 @pytest.mark.parametrize("form", ["bf16", "f32", "f16"])
@@ -118,10 +117,10 @@ def test_mixed_deterministic(ffs, oracle, torch_cuda):
     assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("form", ["bf16", "bf16x2"])
 @pytest.mark.parametrize("tile", [16, 32])
 @pytest.mark.parametrize("size", [(37, 53), (100, 64), (7, 300)])
-def test_2d_bf16_tiles(ffs, oracle, torch_cuda, form, tile, size):
+def test_2d_bf16_tiles(ffs, oracle, torch_cuda, tile, size):
+    form = "bf16"
     _, r, w, _ = ffs.named_stencil("Box2D25P", seed=31)
     x = oracle.uniform(55 + size[0], size)
     for steps in (1, 4, 11):
@@ -129,8 +128,7 @@ def test_2d_bf16_tiles(ffs, oracle, torch_cuda, form, tile, size):
         assert info["tile"][:2] == [tile, tile]
         ref = oracle.sweep(x, r, w, steps)
         _, rel, _ = oracle.errors(got, ref)
-        tol = TOL_MIXED if form == "bf16" else 2e-5  # the 2-term split is only 1e-6-accurate on smoothed fields
-        assert rel <= tol, (form, tile, size, steps, info, rel)
+        assert rel <= TOL_MIXED, (form, tile, size, steps, info, rel)
 
 
 @pytest.mark.parametrize("form", ["bf16", "f32"])
@@ -210,3 +208,37 @@ def test_planner_depth_8192(ffs, torch_cuda, form, shape, depth):
     assert info["tile"][:2] == [32, 32] and info["fuse"] == depth and info["valid"][:2] == [16, 16], info
     # AUTO mapping: XCD bands for the FP64 32-point kernel, hardware order for the mixed forms
     assert info["mapping"] == (1 if form == "fp64" else 2), info
+
+
+def test_bf16x2_form_withdrawn(ffs, torch_cuda):
+    # the 2-term BF16 split misses 1e-6 before the field has smoothed: plan creation refuses it
+    with pytest.raises(ffs.FFSError, match="UNSUPPORTED"):
+        ffs.Plan(2, (64, 64), np.full(9, 1 / 9), 1, 4, precision="bf16x2")
+
+
+@pytest.mark.parametrize("form", ["f16", "bf16", "fp64"])
+@pytest.mark.parametrize("convention", ["ref", "centred"])
+@pytest.mark.parametrize("radius,fuse,size", [(2, 4, (8192, 96)), (1, 8, (100, 70)), (2, 3, (66, 130)),
+                                              (3, 2, (40, 77)), (2, 4, (45, 70))])
+def test_out16_bitwise_equals_full_stages(ffs, oracle, torch_cuda, form, convention, radius, fuse, size):
+    # 32-point tiles with V <= 16: the reduced inverse stages (only the kept output block, from the
+    # shifted inverse operands) must give bitwise the values of the full stages at those positions.
+    # Steps 2 t + 1: a main block and a tail block of another shift.  (3, 2): V = 20 > 16, full both ways.
+    torch = torch_cuda
+    rng = np.random.default_rng(700 + 10 * radius + fuse)
+    w = rng.random((2 * radius + 1) ** 2)
+    w /= w.sum()
+    x = oracle.uniform(800 + fuse, size)
+    d_in = torch.from_numpy(x).cuda()
+    outs = []
+    for full in (False, True):
+        plan = ffs.Plan(2, size, w, radius, 2 * fuse + 1, precision=form, convention=convention, fuse=fuse, tile=32,
+                        full_stages=full)
+        d = torch.empty_like(d_in)
+        plan.run(d_in, d)
+        torch.cuda.synchronize()
+        info = plan.info
+        plan.close()
+        outs.append(d)
+    assert info["valid"][0] == 32 - 2 * radius * fuse
+    assert torch.equal(outs[0], outs[1]), (form, convention, radius, fuse, float((outs[0] - outs[1]).abs().max()))

@@ -216,3 +216,31 @@ def test_planner_heat1d_single_launch(ffs, torch_cuda):
     info = plan.info
     plan.close()
     assert info["fuse"] == 64 and info["blocks"] == 1 and info["valid"][0] == 112, info
+
+
+@pytest.mark.parametrize("precision", ["fp64", "f16"])
+def test_graph_cache_cycles_buffer_pairs(ffs, oracle, torch_cuda, precision):
+    # one plan driven with five (d_in, d_out) pairs in turn, twice round: the plan keeps the captured chains
+    # of four pairs, so the fifth evicts the least recently used one and every pair is re-captured on its
+    # next turn; every run must equal the first run of its input bitwise, and the FP64 ones the oracle
+    torch = torch_cuda
+    _, r, w, _ = ffs.named_stencil("Box2D9P", seed=13)
+    plan = ffs.Plan(2, (96, 80), w, r, 48, precision=precision, fuse=4)
+    assert plan.info["blocks"] > 4  # graph replay is used
+    xs = [oracle.uniform(60 + i, (96, 80)) for i in range(5)]
+    ins = [torch.from_numpy(x).cuda() for x in xs]
+    outs = [torch.empty_like(d) for d in ins]
+    first = []
+    for rnd in range(2):
+        for i in range(5):
+            outs[i].fill_(float("nan"))
+            plan.run(ins[i], outs[i])
+            torch.cuda.synchronize()
+            if rnd == 0:
+                first.append(outs[i].clone())
+            else:
+                assert torch.equal(outs[i], first[i]), i
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    for i in range(5):
+        assert oracle.errors(first[i].cpu().numpy(), oracle.sweep(xs[i], r, w, 48))[1] <= tol
+    plan.close()
