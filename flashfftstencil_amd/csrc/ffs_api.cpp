@@ -248,6 +248,13 @@ ffs_status choose_config(ffs_plan* pl) {
   }
   if (p.precision == FFS_PREC_MIXED_F16 && p.dims != 2)
     return fail(FFS_ERR_UNSUPPORTED, "the FP16 mixed form is offered for 2D grids (32-point tiles) only");
+  // Six FP32-operand stages per block round the accumulator every 4 products; on smooth 3D fields
+  // (Box3D27P 512^3 x 32, sinusoid field) that measured 1.3e-6 > 1e-6.  A compensated form needs two or
+  // more FP32 products per real GEMM, which on MI355X (FP32 matrix rate = 2x FP64) costs as much as the
+  // FP64 form's single product: 3D runs FP64 or the BF16 split.
+  if (p.precision == FFS_PREC_MIXED_F32 && p.dims == 3)
+    return fail(FFS_ERR_UNSUPPORTED, "the FP32 mixed form is offered for 1D and 2D grids; in 3D it misses 1e-6 "
+                                     "(use FFS_PREC_MIXED_BF16 or FFS_PREC_FP64)");
   if (p.dims == 1) {
     if (p.tile != 0 && p.tile != 240)
       return fail(FFS_ERR_UNSUPPORTED, "1D tile must be 240 (15x16 prime-factor window), got %d", p.tile);

@@ -131,7 +131,7 @@ def test_2d_bf16_tiles(ffs, oracle, torch_cuda, tile, size):
         assert rel <= TOL_MIXED, (form, tile, size, steps, info, rel)
 
 
-@pytest.mark.parametrize("form", ["bf16", "f32"])
+@pytest.mark.parametrize("form", ["bf16"])
 @pytest.mark.parametrize("shape", ["Heat-3D", "Box3D27P"])
 @pytest.mark.parametrize("size", [(16, 16, 16), (11, 13, 17), (40, 24, 30)])
 def test_3d_mixed_named(ffs, oracle, torch_cuda, form, shape, size):
@@ -144,7 +144,14 @@ def test_3d_mixed_named(ffs, oracle, torch_cuda, form, shape, size):
         assert rel <= TOL_MIXED, (form, shape, size, steps, info, rel)
 
 
-@pytest.mark.parametrize("form", ["bf16", "f32"])
+def test_3d_f32_form_withdrawn(ffs, torch_cuda):
+    # FP32-operand stages miss 1e-6 on smooth 3D fields (six roundings of the accumulator every 4
+    # products per block): plan creation refuses the form in 3D, and the 3D driver reports it
+    with pytest.raises(ffs.FFSError, match="UNSUPPORTED"):
+        ffs.Plan(3, (16, 16, 16), np.full(27, 1 / 27), 1, 4, precision="f32")
+
+
+@pytest.mark.parametrize("form", ["bf16"])
 def test_3d_mixed_sinusoid_field(ffs, oracle, torch_cuda, form):
     # The 3D workload's field scaled down with the same spectral content: random-phase sinusoids with
     # integer wavevectors up to n/32 per axis (16 on 512^3, 2 on 64^3) plus 1e-3 noise, T = 32.
@@ -161,11 +168,7 @@ def test_3d_mixed_sinusoid_field(ffs, oracle, torch_cuda, form):
     ref = oracle.sweep(x, r, w, 32)
     _, rel, l2 = oracle.errors(got, ref)
     print(form, rel, l2)
-    # The FP32-operand form rounds the accumulator every 4 products (v_mfma_f32_16x16x4f32) through six
-    # stages per block; on smooth 3D fields that lands near 1.4e-6, so its stated 3D bound is 2e-6.
-    # The BF16 form (K = 32 per rounding) meets the mixed-form bar of 1e-6.
-    tol = TOL_MIXED if form == "bf16" else 2e-6
-    assert rel <= tol, (info, rel, l2)
+    assert rel <= TOL_MIXED, (info, rel, l2)
 
 
 @pytest.mark.parametrize("scale", [1e-30, 1e-6, 1.0, 1e6, 1e30])
