@@ -9,7 +9,9 @@ Workloads (seed 0x5EED0000 + index, SplitMix64; weights uniform[0,1) normalised 
   C2 Box2D9P 8192^2 x T=128             FP64 FFT form (+ mixed forms, direct)
   C3 Box2D25P 8192^2 x T=256            mixed FP16 form vs FP64 (+ BF16, direct)  <- `value`
   C4 Box3D27P 512^3 x T=32              sinusoid field; mixed BF16 form vs FP64 (+ direct)
-  C5 96 2D grids, 512..4096 sides, 4 stencils, T 16..256: one persistent launch per round
+  C5 96 2D grids, 512..4096 sides, 4 stencils, T 16..256: one persistent launch per round, fused depths
+     and mapping from the batch autotuner (ffs_batch_autotune, timed outside the timed region; the
+     planner's depths are timed beside it)
 
 `value` is C3's mixed FP16 form (the metric's headline comparison); every workload's record sits under
 `configs` with its forms' GStencils/s, the dominant kernel's roofline, max-rel / rel-L2 against the FP64
@@ -378,7 +380,16 @@ def bench_batch(args, torch, ffs, forms=None):
     res, outs = {}, {}
     for form in forms:
         d_out = [torch.empty_like(d) for d in d_in]
+        # the planner's depths, then the batch autotuner's (whole-batch timings; cached per workload)
         batch = ffs.Batch(specs, precision=form)
+        planned_ms, _ = timed(torch, lambda: batch.run(d_in, d_out), args.warmup, args.steps, stream)
+        planned_fuse = [batch.info(i)["fuse"] for i in range(len(specs))]
+        batch.close()
+        t0 = time.perf_counter()
+        cache = os.path.join(args.tune_dir, f"batch96_{form}.jsonl") if args.tune_dir else None
+        tuned_specs, _ = ffs.Batch.autotune(specs, precision=form, cache=cache)
+        tune_s = time.perf_counter() - t0
+        batch = ffs.Batch(tuned_specs, precision=form)
         ms, reps = timed(torch, lambda: batch.run(d_in, d_out), args.warmup, args.steps, stream)
         gst = pts / (ms * 1e-3) / 1e9
         flops = bytes_alg = 0.0
@@ -397,11 +408,15 @@ def bench_batch(args, torch, ffs, forms=None):
         else:
             rl.update(bound="mfma", achieved=flops / s / 1e12, peak=MATRIX_PEAK[form] / 1e12, unit="TFLOP/s",
                       frac=f_frac)
-        res[form] = dict(value=gst, ms_per_step=ms, timed_runs=reps, roofline=rl, rounds=rounds)
+        changed = sum(1 for i in range(len(specs)) if batch.info(i)["fuse"] != planned_fuse[i])
+        res[form] = dict(value=gst, ms_per_step=ms, timed_runs=reps, roofline=rl, rounds=rounds,
+                         autotune={"tune_s": tune_s, "planned_ms": planned_ms,
+                                   "planned_value": pts / (planned_ms * 1e-3) / 1e9, "grids_retuned": changed,
+                                   "mapping": tuned_specs[0]["mapping"]})
         outs[form] = d_out
         batch.close()
         log(f"batch96 {form}: {gst:.1f} GStencil/s, {ms:.2f} ms/run, {rounds} rounds, roofline {rl['bound']} "
-            f"{rl['frac']:.1%}")
+            f"{rl['frac']:.1%}; planner depths {planned_ms:.2f} ms/run, autotune {tune_s:.1f} s changed {changed} grids")
     if "fp64" in outs:  # per-grid error of the mixed forms against the batched FP64 form
         for form in outs:
             if form == "fp64":
@@ -457,6 +472,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines and the oracle sample checks")
     ap.add_argument("--forms", default=None, help="comma list overriding the workloads' forms")
     ap.add_argument("--no-fft-baseline", action="store_true", help="skip the rocFFT and library-convolution baselines")
+    ap.add_argument("--tune-dir", default=None, help="directory for the batch autotuner's cache files (default: none, "
+                                                     "tune every run)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 plain device copies of the grid (known bytes) to calibrate HBM counters")
     args = ap.parse_args()

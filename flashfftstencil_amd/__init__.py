@@ -142,6 +142,7 @@ def load_library() -> ctypes.CDLL:
         "ffs_batch_run": [ctypes.c_void_p, P(ctypes.c_void_p), P(ctypes.c_void_p), ctypes.c_void_p],
         "ffs_batch_get_info": [ctypes.c_void_p, ctypes.c_int32, P(PlanInfo)],
         "ffs_batch_destroy": [ctypes.c_void_p],
+        "ffs_batch_autotune": [P(Problem), ctypes.c_int32, ctypes.c_char_p, P(ctypes.c_double)],
     }
     for name, argtypes in sigs.items():
         fn = getattr(lib, name)
@@ -345,6 +346,33 @@ def autotune_full(dims: int, sizes: Sequence[int], weights, radius: int, steps: 
     return name, p.tile, p.fuse, mname, ms.value
 
 
+def _batch_problems(problems, precision):
+    """ctypes array of ffs_problem for a batch (2D, tile 32), the weight arrays it points to, and the sizes."""
+    lib = load_library()
+    n = len(problems)
+    arr = (Problem * n)()
+    ws, sizes = [], []
+    for i, pr in enumerate(problems):
+        p = arr[i]
+        _check(lib.ffs_problem_init(ctypes.byref(p)))
+        w = np.ascontiguousarray(np.asarray(pr["weights"], dtype=np.float64).ravel())
+        ws.append(w)
+        p.dims = 2
+        p.radius = int(pr["radius"])
+        p.sizes[0], p.sizes[1] = int(pr["sizes"][0]), int(pr["sizes"][1])
+        p.weights = _dptr(w)
+        p.steps = int(pr["steps"])
+        prec = pr.get("precision", precision)
+        p.precision = PRECISIONS[prec] if isinstance(prec, str) else int(prec)
+        conv = pr.get("convention", "ref")
+        p.convention = CONVENTIONS[conv] if isinstance(conv, str) else int(conv)
+        p.fuse = int(pr.get("fuse", 0))
+        mp = pr.get("mapping", "auto")
+        p.mapping = MAPPINGS[mp] if isinstance(mp, str) else int(mp)
+        sizes.append((int(pr["sizes"][0]), int(pr["sizes"][1])))
+    return arr, ws, sizes
+
+
 class Batch:
     """Independent 2D grids advanced together (one persistent launch per fused block; FP64, FP16 or BF16).
 
@@ -353,31 +381,30 @@ class Batch:
 
     def __init__(self, problems, precision: str = "bf16"):
         lib = load_library()
+        arr, self._w, self.sizes = _batch_problems(problems, precision)
         n = len(problems)
-        arr = (Problem * n)()
-        self._w = []
-        self.sizes = []
-        for i, pr in enumerate(problems):
-            p = arr[i]
-            _check(lib.ffs_problem_init(ctypes.byref(p)))
-            w = np.ascontiguousarray(np.asarray(pr["weights"], dtype=np.float64).ravel())
-            self._w.append(w)
-            p.dims = 2
-            p.radius = int(pr["radius"])
-            p.sizes[0], p.sizes[1] = int(pr["sizes"][0]), int(pr["sizes"][1])
-            p.weights = _dptr(w)
-            p.steps = int(pr["steps"])
-            prec = pr.get("precision", precision)
-            p.precision = PRECISIONS[prec] if isinstance(prec, str) else int(prec)
-            conv = pr.get("convention", "ref")
-            p.convention = CONVENTIONS[conv] if isinstance(conv, str) else int(conv)
-            p.fuse = int(pr.get("fuse", 0))
-            self.sizes.append((int(pr["sizes"][0]), int(pr["sizes"][1])))
         h = ctypes.c_void_p()
         _check(lib.ffs_batch_create(arr, n, ctypes.byref(h)))
         self._h = h
         self.count = n
         self.device = 0
+
+    @staticmethod
+    def autotune(problems, precision: str = "bf16", cache: Optional[str] = None):
+        """ffs_batch_autotune: copies of `problems` with the tuned fuse (and mapping) of every grid, and the
+        tuned batch's ms per run."""
+        arr, keep, _ = _batch_problems(problems, precision)
+        ms = ctypes.c_double()
+        _check(load_library().ffs_batch_autotune(arr, len(problems), cache.encode() if cache else None,
+                                                 ctypes.byref(ms)))
+        out = []
+        for pr, p in zip(problems, arr):
+            q = dict(pr)
+            q["fuse"] = int(p.fuse)
+            q["mapping"] = {v: k for k, v in MAPPINGS.items()}[int(p.mapping)]
+            out.append(q)
+        del keep
+        return out, ms.value
 
     def info(self, i: int) -> dict:
         inf = PlanInfo()

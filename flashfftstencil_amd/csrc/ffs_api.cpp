@@ -1317,6 +1317,221 @@ ffs_status ffs_batch_destroy(ffs_batch* b) {
   return FFS_OK;
 }
 
+ffs_status ffs_batch_autotune(ffs_problem* problems, int32_t count, const char* cache_path, double* best_ms) {
+  if (!problems || count <= 0) return fail(FFS_ERR_INVALID_ARG, "batch needs at least one problem");
+  // Depth classes: grids of one radius move together.  Base depth of each grid: the planner's own choice
+  // on 32-point tiles (fuse = 0), or the caller's fuse when one is set.
+  std::vector<int64_t> base((size_t)count), tmax((size_t)count);
+  std::vector<int32_t> cls((size_t)count);
+  std::vector<int32_t> radii;
+  for (int32_t i = 0; i < count; i++) {
+    ffs_problem q = problems[i];
+    q.tile = 32;
+    ffs_plan probe;
+    ffs_status st = plan_host(&q, &probe);
+    if (st != FFS_OK) return st;
+    base[(size_t)i] = probe.t;
+    const int r = q.radius;
+    tmax[(size_t)i] = r == 0 ? 1 : std::max<int64_t>(1, std::min<int64_t>((32 - 2) / (2 * r), std::max<int64_t>(q.steps, 1)));
+    auto it = std::find(radii.begin(), radii.end(), r);
+    if (it == radii.end()) {
+      radii.push_back(r);
+      cls[(size_t)i] = (int32_t)radii.size() - 1;
+    } else {
+      cls[(size_t)i] = (int32_t)(it - radii.begin());
+    }
+  }
+  // Key: every field the batch's timing depends on, hashed (FNV-1a) over all problems, and the device.
+  uint64_t h = 0xcbf29ce484222325ULL;
+  auto mix = [&](const void* data, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(data);
+    for (size_t k = 0; k < n; k++) h = (h ^ c[k]) * 0x100000001b3ULL;
+  };
+  for (int32_t i = 0; i < count; i++) {
+    const ffs_problem& q = problems[i];
+    const int32_t f[7] = {q.dims, q.radius, q.precision, q.convention, q.flags, q.fuse, q.mapping};
+    mix(f, sizeof f);
+    mix(q.sizes, sizeof q.sizes);
+    mix(&q.steps, sizeof q.steps);
+    int64_t ntaps = 1;
+    for (int d = 0; d < q.dims; d++) ntaps *= 2 * q.radius + 1;
+    mix(q.weights, sizeof(double) * (size_t)ntaps);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(FFS_ERR_NO_DEVICE, "no HIP device visible");
+  const int dev = problems[0].device;
+  if (dev < 0 || dev >= ndev) return fail(FFS_ERR_INVALID_ARG, "device %d out of range", dev);
+  hipDeviceProp_t prop;
+  FFS_HIP(hipGetDeviceProperties(&prop, dev));
+  char key[96];
+  std::snprintf(key, sizeof key, "batch%d_%016llx_%.*s", count, (unsigned long long)h, 24, prop.gcnArchName);
+  for (char* c = key; *c; c++)
+    if (*c == ':' || *c == '"' || *c == ' ') *c = '.';
+  auto apply = [&](const std::vector<int32_t>& fuse, int32_t mapping) {
+    for (int32_t i = 0; i < count; i++) {
+      problems[i].tile = 32;
+      problems[i].algorithm = FFS_ALGO_FFT;
+      problems[i].fuse = fuse[(size_t)i];
+      problems[i].mapping = mapping;
+    }
+  };
+  // a line: {"key": "...", "mapping": M, "ms": x, "fuse": [t0, t1, ...]}
+  if (cache_path) {
+    if (FILE* f = std::fopen(cache_path, "r")) {
+      std::string line;
+      std::vector<int32_t> got;
+      int32_t mapping = 0;
+      double ms = 0.0;
+      bool found = false;
+      for (int ch; (ch = std::fgetc(f)) != EOF || !line.empty();) {
+        if (ch != '\n' && ch != EOF) {
+          line.push_back((char)ch);
+          continue;
+        }
+        char k[96];
+        int mp = 0;
+        double m = 0.0;
+        int used = 0;
+        if (std::sscanf(line.c_str(), " {\"key\": \"%95[^\"]\", \"mapping\": %d, \"ms\": %lf, \"fuse\": [%n", k, &mp, &m,
+                        &used) == 3 && used > 0 && std::strcmp(k, key) == 0) {
+          std::vector<int32_t> v;
+          const char* c = line.c_str() + used;
+          while (*c) {
+            char* end = nullptr;
+            const long x = std::strtol(c, &end, 10);
+            if (end == c) break;
+            v.push_back((int32_t)x);
+            c = end;
+            while (*c == ',' || *c == ' ') c++;
+          }
+          if ((int32_t)v.size() == count) {
+            got = v;
+            mapping = mp;
+            ms = m;
+            found = true;
+          }
+        }
+        line.clear();
+        if (ch == EOF) break;
+      }
+      std::fclose(f);
+      bool ok = found;
+      for (int32_t i = 0; ok && i < count; i++) ok = got[(size_t)i] >= 1 && got[(size_t)i] <= tmax[(size_t)i];
+      if (ok) {  // else: stale or foreign line, tune again
+        apply(got, mapping);
+        if (best_ms) *best_ms = ms;
+        return FFS_OK;
+      }
+    }
+  }
+  // Device grids for the timing runs (the caller's sizes, seeded data like the caller's).
+  FFS_HIP(hipSetDevice(dev));
+  std::vector<double*> din((size_t)count, nullptr), dout((size_t)count, nullptr);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&]() {
+    for (double* d : din)
+      if (d) (void)hipFree(d);
+    for (double* d : dout)
+      if (d) (void)hipFree(d);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  for (int32_t i = 0; i < count && e == hipSuccess; i++) {
+    int64_t n = 1;
+    for (int d = 0; d < problems[i].dims; d++) n *= problems[i].sizes[d];
+    e = hipMalloc(&din[(size_t)i], sizeof(double) * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc(&dout[(size_t)i], sizeof(double) * (size_t)n);
+    if (e == hipSuccess) e = ffsk::fill_uniform_device(din[(size_t)i], n, 0x5EED0000ULL + (uint64_t)i, nullptr);
+  }
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(FFS_ERR_OUT_OF_MEMORY, "batch autotune grids: %s", hipGetErrorString(e));
+  }
+  // Time one configuration: a whole batch run (all rounds from its hipGraph), best of 3 after a warm-up.
+  std::vector<ffs_problem> trial(problems, problems + count);
+  auto timed = [&](const std::vector<int32_t>& fuse, int32_t mapping, double* ms_out) -> ffs_status {
+    for (int32_t i = 0; i < count; i++) {
+      trial[(size_t)i] = problems[i];
+      trial[(size_t)i].tile = 32;
+      trial[(size_t)i].algorithm = FFS_ALGO_FFT;
+      trial[(size_t)i].fuse = fuse[(size_t)i];
+      trial[(size_t)i].mapping = mapping;
+    }
+    ffs_batch* b = nullptr;
+    ffs_status st = ffs_batch_create(trial.data(), count, &b);
+    if (st != FFS_OK) return st;
+    double best = 1e300;
+    for (int rep = 0; rep < 4 && st == FFS_OK; rep++) {  // rep 0 captures the graph and warms up
+      (void)hipEventRecord(e0, nullptr);
+      st = ffs_batch_run(b, din.data(), dout.data(), nullptr);
+      (void)hipEventRecord(e1, nullptr);
+      if (hipEventSynchronize(e1) != hipSuccess) st = fail(FFS_ERR_HIP, "batch autotune: run failed");
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (rep > 0) best = std::min(best, (double)ms);
+    }
+    ffs_batch_destroy(b);
+    *ms_out = best;
+    return st;
+  };
+  std::vector<int32_t> cur((size_t)count);
+  for (int32_t i = 0; i < count; i++)
+    cur[(size_t)i] = (int32_t)std::max<int64_t>(1, std::min(base[(size_t)i], tmax[(size_t)i]));
+  // Mapping: the one the caller set, else both orders are tried.
+  std::vector<int32_t> maps;
+  if (problems[0].mapping != FFS_MAP_AUTO) maps = {problems[0].mapping};
+  else maps = {FFS_MAP_LINEAR, FFS_MAP_XCD};
+  int32_t best_map = maps[0];
+  double best = 1e300;
+  for (int32_t mp : maps) {
+    double ms = 0.0;
+    ffs_status st = timed(cur, mp, &ms);
+    if (st != FFS_OK) {
+      cleanup();
+      return st;
+    }
+    if (ms < best) {
+      best = ms;
+      best_map = mp;
+    }
+  }
+  // Coordinate descent over the depth classes: shift every grid of one radius by -2 .. +2 (clamped to
+  // its admissible range) and keep a shift that makes the whole batch faster.
+  for (size_t c = 0; c < radii.size(); c++) {
+    std::vector<int32_t> keep = cur;
+    for (int delta : {-2, -1, 1, 2}) {
+      std::vector<int32_t> cand = keep;
+      bool changed = false;
+      for (int32_t i = 0; i < count; i++) {
+        if (cls[(size_t)i] != (int32_t)c) continue;
+        const int64_t t = std::min<int64_t>(std::max<int64_t>(keep[(size_t)i] + delta, 1), tmax[(size_t)i]);
+        changed |= t != keep[(size_t)i];
+        cand[(size_t)i] = (int32_t)t;
+      }
+      if (!changed) continue;
+      double ms = 0.0;
+      if (timed(cand, best_map, &ms) == FFS_OK && ms < best) {
+        best = ms;
+        cur = cand;
+      }
+    }
+  }
+  cleanup();
+  apply(cur, best_map);
+  if (best_ms) *best_ms = best;
+  if (cache_path) {
+    if (FILE* f = std::fopen(cache_path, "a")) {
+      std::fprintf(f, "{\"key\": \"%s\", \"mapping\": %d, \"ms\": %.6f, \"fuse\": [", key, best_map, best);
+      for (int32_t i = 0; i < count; i++) std::fprintf(f, "%s%d", i ? ", " : "", cur[(size_t)i]);
+      std::fprintf(f, "]}\n");
+      std::fclose(f);
+    }
+  }
+  return FFS_OK;
+}
+
 const char* ffs_last_error(void) { return g_err.c_str(); }
 const char* ffs_version(void) { return FFS_VERSION_STRING; }
 

@@ -216,6 +216,19 @@ ffs_status ffs_batch_run(ffs_batch* batch, const double* const* d_in, double* co
 ffs_status ffs_batch_get_info(const ffs_batch* batch, int32_t index, ffs_plan_info* info);
 ffs_status ffs_batch_destroy(ffs_batch* batch);
 
+/* Batch autotuner: picks the fused depth of every problem and the batch's wave-to-tile mapping by timing
+ * whole batch runs (every round of the batch, from its hipGraph, on seeded grids of the problems' sizes),
+ * since a grid's blocks share launches with the other grids and its standalone timing would count launch
+ * costs it does not pay in the batch.  Start: the planner's depth of each grid on 32-point tiles (or
+ * the caller's fuse); then, for each radius in turn, every grid of that radius is shifted by -2..+2
+ * depths (within its admissible range) and a shift is kept when the whole batch gets faster; both
+ * mappings are tried unless problems[0].mapping is set.  Writes tile = 32, fuse, mapping and
+ * algorithm = FFT into every problem, ready for ffs_batch_create.  With a non-NULL `cache_path` the result
+ * is read from / appended to that JSON-lines file, keyed by a hash of all problems (sizes, T, radius,
+ * weights, precision, convention, flags) and the device architecture.  best_ms (may be NULL) receives
+ * the tuned batch's time per run.  No reference counterpart (the reference runs one grid per process). */
+ffs_status ffs_batch_autotune(ffs_problem* problems, int32_t count, const char* cache_path, double* best_ms);
+
 /* Reference-format report line:
  *   "FlashFFTStencil, <tag>, <d1>, <d2>, <d3>, <T>, <ms>, <GStencil/s>"
  * with leading 1s for unused dimensions and GStencil/s = points * T / ms / 1e6. */

@@ -85,3 +85,36 @@ def test_batch_rejects_mixed_precisions(ffs):
              dict(radius=r, weights=w, sizes=(32, 32), steps=2, precision="f16")]
     with pytest.raises(ffs.FFSError):
         ffs.Batch(specs)
+
+
+@pytest.mark.parametrize("form", ["f16", "fp64"])
+def test_batch_autotune(ffs, oracle, torch_cuda, form, tmp_path):
+    # the batch tuner times whole batch runs; the depths it returns must be admissible (each batch
+    # creates), the tuned batch must meet the form's bar against the oracle and equal single plans of
+    # the same depths bitwise, and a second call must come from the cache file (same answer, one line)
+    torch = torch_cuda
+    specs = _specs(ffs, 11)
+    cache = str(tmp_path / "batch_tune.jsonl")
+    tuned, ms = ffs.Batch.autotune(specs, precision=form, cache=cache)
+    assert ms > 0 and len(tuned) == len(specs)
+    again, ms2 = ffs.Batch.autotune(specs, precision=form, cache=cache)
+    assert [t["fuse"] for t in again] == [t["fuse"] for t in tuned] and ms2 == pytest.approx(ms, rel=1e-5)
+    assert len(open(cache).readlines()) == 1
+    xs = [oracle.uniform(900 + i, s["sizes"]) for i, s in enumerate(tuned)]
+    d_in = [torch.from_numpy(x).cuda() for x in xs]
+    d_out = [torch.empty_like(d) for d in d_in]
+    batch = ffs.Batch(tuned, precision=form)
+    batch.run(d_in, d_out)
+    torch.cuda.synchronize()
+    for i, s in enumerate(tuned):
+        assert 1 <= s["fuse"] <= max(1, min(15 // s["radius"], s["steps"]))
+        ref = oracle.sweep(xs[i], s["radius"], s["weights"], s["steps"])
+        assert oracle.errors(d_out[i].cpu().numpy(), ref)[1] <= (1e-12 if form == "fp64" else 1e-6), (i, s["fuse"])
+        plan = ffs.Plan(2, s["sizes"], s["weights"], s["radius"], s["steps"], precision=form, fuse=s["fuse"], tile=32,
+                        mapping=s["mapping"])
+        single = torch.empty_like(d_in[i])
+        plan.run(d_in[i], single)
+        torch.cuda.synchronize()
+        plan.close()
+        assert torch.equal(single, d_out[i]), (i, s["fuse"])
+    batch.close()
