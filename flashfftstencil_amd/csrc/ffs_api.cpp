@@ -80,6 +80,7 @@ struct ffs_plan {
   double* d_ops = nullptr;
   uint32_t* d_opsm = nullptr;  // mixed-form operand table
   double* d_H = nullptr;       // [main | tail], each 2*len doubles
+  float* d_Hf = nullptr;       // mixed forms: [main | tail] FP32 copies, each len (re, im) pairs
   double dc_main = 1.0, dc_tail = 1.0;  // K(0)^t and K(0)^t_tail
   int32_t scale_main = 0, scale_tail = 0;  // FP16 form: scale exponents of the main and tail depths
   double* d_taps = nullptr;
@@ -384,6 +385,7 @@ void release(ffs_plan* pl) {
   if (pl->d_opsm) (void)hipFree(pl->d_opsm);
   if (pl->d_opsi) (void)hipFree(pl->d_opsi);
   if (pl->d_H) (void)hipFree(pl->d_H);
+  if (pl->d_Hf) (void)hipFree(pl->d_Hf);
   if (pl->d_taps) (void)hipFree(pl->d_taps);
   if (pl->d_scratch) (void)hipFree(pl->d_scratch);
   delete pl;
@@ -411,6 +413,7 @@ ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, boo
   a.dst = dst;
   a.ops = pl->d_ops;
   a.H = pl->d_H + (tail ? 2 * pl->hlen : 0);
+  a.Hf = pl->d_Hf ? pl->d_Hf + (tail ? 2 * pl->hlen : 0) : nullptr;
   a.opsm = pl->d_opsm;
   a.dc_gain = tail ? pl->dc_tail : pl->dc_main;
   const ffs_problem& p = pl->prob;
@@ -607,7 +610,7 @@ ffs_status plan_constants(ffs_plan* pl) {
 
 // Bytes of each device table of a transform plan (0 = not used).
 struct TableSizes {
-  size_t ops, opsm, H, opsi, taps;
+  size_t ops, opsm, H, Hf, opsi, taps;
 };
 TableSizes table_sizes(const ffs_plan* pl) {
   TableSizes t{};
@@ -615,6 +618,7 @@ TableSizes table_sizes(const ffs_plan* pl) {
   t.ops = sizeof(double) * std::max(ffsk::kOpsLen, ffsk::kOps32F64);
   t.opsm = pl->prob.precision != FFS_PREC_FP64 ? sizeof(uint32_t) * std::max(ffsk::kOpsMixedLen, ffsk::kOps32Words) : 0;
   t.H = sizeof(double) * 4 * (size_t)ffsk::spectrum_len(pl->kind);
+  t.Hf = pl->prob.precision != FFS_PREC_FP64 ? sizeof(float) * 4 * (size_t)ffsk::spectrum_len(pl->kind) : 0;
   t.opsi = out16_block(pl) ? 2 * std::max(sizeof(double) * ffsk::kInv32F64, sizeof(uint32_t) * ffsk::kInv32Words) : 0;
   t.taps = sizeof(double) * pl->weights.size();
   return t;
@@ -650,6 +654,10 @@ hipError_t plan_tables(ffs_plan* pl, hipStream_t s) {
   chk(ffsk::gen_spectrum(pl->d_H, pl->kind, layout, pl->L, pl->d_taps, K, p.dims, p.convention, p.radius, pl->t, s));
   chk(ffsk::gen_spectrum(pl->d_H + 2 * pl->hlen, pl->kind, layout, pl->L, pl->d_taps, K, p.dims, p.convention,
                          p.radius, pl->t_tail > 0 ? pl->t_tail : pl->t, s));
+  if (pl->d_Hf) {
+    chk(ffsk::spectrum_to_f32(pl->d_H, pl->d_Hf, pl->hlen, s));
+    chk(ffsk::spectrum_to_f32(pl->d_H + 2 * pl->hlen, pl->d_Hf + 2 * pl->hlen, pl->hlen, s));
+  }
   return e;
 }
 
@@ -766,6 +774,7 @@ ffs_status ffs_plan_create(const ffs_problem* p, ffs_plan** out) {
     chk(hipMalloc(&pl->d_ops, ts.ops));
     if (ts.opsm) chk(hipMalloc(&pl->d_opsm, ts.opsm));
     chk(hipMalloc(&pl->d_H, ts.H));
+    if (ts.Hf) chk(hipMalloc(&pl->d_Hf, ts.Hf));
     if (ts.opsi) chk(hipMalloc(&pl->d_opsi, ts.opsi));
     chk(hipMalloc(&pl->d_taps, ts.taps));
   }

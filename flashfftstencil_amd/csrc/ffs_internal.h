@@ -51,6 +51,7 @@ struct TileArgs {
   double* dst;
   const double* ops;  // DFT operands (doubles)
   const double* H;    // spectrum^t / (P0 P1 [P2]) for this block, family layout, [re plane | im plane]
+  const float* Hf;    // mixed forms: the same spectrum rounded to FP32, (re, im) pairs (one 8-byte load each)
   const uint32_t* opsm;  // mixed-form operand table (kOpsMixedLen words)
   int64_t n0, n1, n2;       // grid sides (unused axes = 1); 1D uses n1
   int64_t tiles0, tiles1, tiles2;
@@ -113,6 +114,9 @@ hipError_t fill_uniform_device(double* d, int64_t n, uint64_t seed, hipStream_t 
 hipError_t gen_spectrum(double* H, int kind, int layout, const int32_t* axis_len, const double* d_taps,
                         int32_t ntaps_per_axis, int32_t dims, int32_t convention, int32_t radius, int64_t t,
                         hipStream_t s);
+// Hf[2e], Hf[2e + 1] = (float)H[e], (float)H[len + e] for e < len: the FP32 copy of one depth's spectrum
+// that the mixed-form kernels read from global memory (the batch kernels and the 3D kernel).
+hipError_t spectrum_to_f32(const double* H, float* Hf, int64_t len, hipStream_t s);
 hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus);
 // One fused block of every grid of a batch (32-point family: FP64, BF16, FP16), items/prefix in device memory.
 hipError_t launch_batch(const TileArgs* d_items, const int64_t* d_prefix, int32_t n, int32_t precision, hipStream_t s,

@@ -1042,8 +1042,8 @@ __device__ __forceinline__ void tile32_pair(const TileArgs& a, int64_t tp, const
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int e = (b * 4 + v) * kLanes + lane;
-      const float hr = H_FROM_LDS ? sH[e][0] : (float)a.H[e];
-      const float hi = H_FROM_LDS ? sH[e][1] : (float)a.H[1024 + e];
+      const f2 hv = H_FROM_LDS ? sH[e] : reinterpret_cast<const f2*>(a.Hf)[e];
+      const float hr = hv[0], hi = hv[1];
       float yr = zr.b[b][v], yi = zi.b[b][v];
       cmul(yr, yi, hr, hi);
       zr.b[b][v] = yr;
@@ -1222,8 +1222,8 @@ __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, c
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int ee = (b * 4 + v) * kLanes + lane;
-      const float hr = H_FROM_LDS ? sH[ee][0] : (float)a.H[ee];
-      const float hi = H_FROM_LDS ? sH[ee][1] : (float)a.H[1024 + ee];
+      const f2 hv = H_FROM_LDS ? sH[ee] : reinterpret_cast<const f2*>(a.Hf)[ee];
+      const float hr = hv[0], hi = hv[1];
       float yr = zr.b[b][v], yi = zi.b[b][v];
       cmul(yr, yi, hr, hi);
       zr.b[b][v] = yr;
@@ -1552,9 +1552,18 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
       P::template stage<false>(pr[s], pi[s], f, br, bi);  // [k1 = row][k0 = lane]
       // H entry (k2, v, lane): k0 = j, k1 = rowmap(q, v), k2
       const double* H = a.H + (int64_t)k2 * 4 * kLanes + lane;
+      const f2* Hf = reinterpret_cast<const f2*>(a.Hf) + (int64_t)k2 * 4 * kLanes + lane;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const V hr = (V)H[v * kLanes], hi = (V)H[4096 + v * kLanes];
+        V hr, hi;
+        if (mixed) {  // the FP32 copy: one 8-byte load instead of two FP64 loads and conversions
+          const f2 hv = Hf[v * kLanes];
+          hr = (V)hv[0];
+          hi = (V)hv[1];
+        } else {
+          hr = (V)H[v * kLanes];
+          hi = (V)H[4096 + v * kLanes];
+        }
         V yr = br[v], yi = bi[v];
         cmul(yr, yi, hr, hi);
         br[v] = yr;
@@ -1909,6 +1918,19 @@ hipError_t gen_spectrum(double* H, int kind, int layout, const int32_t* axis_len
   const int len = (int)spectrum_len(kind);
   hipLaunchKernelGGL(k_gen_spectrum, dim3((len + 255) / 256), dim3(256), 0, s, H, kind, layout, axis_len[0],
                      axis_len[1], axis_len[2], d_taps, K, dims, convention, radius, t);
+  return hipGetLastError();
+}
+
+__global__ void k_spectrum_f32(const double* H, float* Hf, int64_t len) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < len) {
+    Hf[2 * e] = (float)H[e];
+    Hf[2 * e + 1] = (float)H[len + e];
+  }
+}
+
+hipError_t spectrum_to_f32(const double* H, float* Hf, int64_t len, hipStream_t s) {
+  hipLaunchKernelGGL(k_spectrum_f32, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, H, Hf, len);
   return hipGetLastError();
 }
 
