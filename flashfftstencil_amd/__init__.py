@@ -133,6 +133,7 @@ def load_library() -> ctypes.CDLL:
         "ffs_run": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
         "ffs_run_host": [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double)],
         "ffs_run_block": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p],
+        "ffs_run_static": [P(Problem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
         "ffs_format_csv": [ctypes.c_char_p, ctypes.c_int32, P(ctypes.c_int64), ctypes.c_int64, ctypes.c_double,
                            ctypes.c_char_p, ctypes.c_int32],
         "ffs_autotune": [P(Problem), ctypes.c_char_p, P(ctypes.c_double)],
@@ -314,6 +315,24 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+STATIC_SCRATCH_BYTES = 64 << 20  # FFS_STATIC_SCRATCH_BYTES
+STATIC_TABLE_BYTES = 1 << 20     # FFS_STATIC_TABLE_BYTES
+
+
+def run_static(dims: int, sizes: Sequence[int], weights, radius: int, steps: int, d_in, d_out, stream=None,
+               **kw) -> None:
+    """ffs_run_static: d_out = S^T(d_in) without allocating device memory (plan tables and ping-pong grid
+    in the library's static per-GPU array).  Keyword arguments as for :class:`Plan`.  Calls that overlap
+    on one GPU race on that array."""
+    w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).ravel())
+    p = _problem(dims, sizes, w, radius, steps, **kw)
+    n = int(np.prod([int(x) for x in sizes][:dims]))
+    dev = kw.get("device", 0)
+    pi, po = _ptr(d_in, n, dev), _ptr(d_out, n, dev)
+    _check(load_library().ffs_run_static(ctypes.byref(p), ctypes.c_void_p(pi), ctypes.c_void_p(po),
+                                         ctypes.c_void_p(_stream_handle(stream))))
 
 
 def autotune(dims: int, sizes: Sequence[int], weights, radius: int, steps: int, precision: str = "fp64",

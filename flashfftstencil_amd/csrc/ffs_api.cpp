@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -661,6 +662,37 @@ hipError_t plan_tables(ffs_plan* pl, hipStream_t s) {
   return e;
 }
 
+// ---- device scratch of the allocation-free entry point (ffs_run_static) ----
+// One static array per GPU: the plan tables in the first FFS_STATIC_TABLE_BYTES, the ping-pong grid in the
+// rest.  In host code `scratch` is only a host-side stand-in for the symbol; its device address on each
+// GPU comes from hipGetSymbolAddress, resolved once per device with that device current and kept in
+// g_static_ptr, and only that address is handed to kernels.
+constexpr int kStaticMaxDevices = 64;
+std::mutex g_static_mu;
+void* g_static_ptr[kStaticMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[FFS_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+// Device address of `scratch` on device `dev` (which must be current).
+ffs_status static_scratch(int dev, void** out) {
+  if (dev < 0 || dev >= kStaticMaxDevices) return fail(FFS_ERR_UNSUPPORTED, "device %d: static scratch is kept for "
+                                                                            "devices 0..%d", dev, kStaticMaxDevices - 1);
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FFS_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev, hipGetErrorString(e));
+    g_static_ptr[dev] = p;
+  }
+  *out = g_static_ptr[dev];
+  return FFS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -973,6 +1005,60 @@ ffs_status ffs_run_block(ffs_plan* pl, const double* d_in, double* d_out, int32_
   hipError_t e = launch_one_block(pl, d_in, d_out, use_tail, (hipStream_t)stream);
   if (e != hipSuccess) return fail(FFS_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
   return FFS_OK;
+}
+
+ffs_status ffs_run_static(const ffs_problem* p, const double* d_in, double* d_out, void* stream) {
+  if (!p || !d_in || !d_out) return fail(FFS_ERR_INVALID_ARG, "NULL argument");
+  if (d_in == d_out) return fail(FFS_ERR_INVALID_ARG, "d_in must not alias d_out");
+  // A host-side plan whose device tables and ping-pong grid are carved out of the static array; it owns
+  // no device memory, so it is never passed to release().
+  ffs_plan pl;
+  ffs_status st = plan_host(p, &pl);
+  if (st == FFS_OK) st = plan_device(&pl);
+  if (st == FFS_OK) st = plan_constants(&pl);
+  if (st != FFS_OK) return st;
+  const TableSizes ts = table_sizes(&pl);
+  size_t off = 0;
+  auto carve = [&](size_t n) {
+    const size_t at = off;
+    off += (n + 255) & ~size_t(255);
+    return at;
+  };
+  const size_t o_ops = carve(ts.ops), o_opsm = carve(ts.opsm), o_H = carve(ts.H), o_Hf = carve(ts.Hf),
+               o_opsi = carve(ts.opsi), o_taps = carve(ts.taps);
+  if (off > FFS_STATIC_TABLE_BYTES)
+    return fail(FFS_ERR_INTERNAL, "plan tables need %zu bytes of the %llu reserved", off, FFS_STATIC_TABLE_BYTES);
+  const size_t grid_bytes = pl.nblocks >= 2 ? sizeof(double) * (size_t)pl.npoints : 0;
+  if (grid_bytes > FFS_STATIC_SCRATCH_BYTES - FFS_STATIC_TABLE_BYTES)
+    return fail(FFS_ERR_UNSUPPORTED,
+                "ffs_run_static: %lld points need %zu bytes of static scratch for the ping-pong grid, the limit is "
+                "%llu (%llu points); use ffs_plan_create + ffs_run",
+                (long long)pl.npoints, grid_bytes, FFS_STATIC_SCRATCH_BYTES - FFS_STATIC_TABLE_BYTES,
+                (FFS_STATIC_SCRATCH_BYTES - FFS_STATIC_TABLE_BYTES) / sizeof(double));
+  void* base = nullptr;
+  st = static_scratch(pl.dev, &base);
+  if (st != FFS_OK) return st;
+  char* b = static_cast<char*>(base);
+  if (!pl.direct) {
+    pl.d_ops = reinterpret_cast<double*>(b + o_ops);
+    pl.d_opsm = ts.opsm ? reinterpret_cast<uint32_t*>(b + o_opsm) : nullptr;
+    pl.d_H = reinterpret_cast<double*>(b + o_H);
+    pl.d_Hf = ts.Hf ? reinterpret_cast<float*>(b + o_Hf) : nullptr;
+    pl.d_opsi = ts.opsi ? static_cast<void*>(b + o_opsi) : nullptr;
+    pl.d_taps = reinterpret_cast<double*>(b + o_taps);
+  }
+  pl.d_scratch = grid_bytes ? reinterpret_cast<double*>(b + FFS_STATIC_TABLE_BYTES) : nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  if (!pl.direct) {
+    hipError_t e = ffsk::put_taps(pl.d_taps, pl.weights.data(), (int64_t)pl.weights.size(), s);
+    if (e == hipSuccess) e = plan_tables(&pl, s);
+    if (e != hipSuccess) return fail(FFS_ERR_HIP, "table generation failed: %s", hipGetErrorString(e));
+  }
+  if (pl.nblocks == 0) {
+    FFS_HIP(hipMemcpyAsync(d_out, d_in, sizeof(double) * pl.npoints, hipMemcpyDeviceToDevice, s));
+    return FFS_OK;
+  }
+  return enqueue_chain(&pl, d_in, d_out, s);
 }
 
 ffs_status ffs_run_host(ffs_plan* pl, const double* h_in, double* h_out) {

@@ -108,6 +108,10 @@ constexpr int kInv32Words = 3 * 3 * 64 * 4;   // 2304 words
 constexpr int kInv32F64 = 2 * 4 * 3 * 64;     // 1536 doubles
 hipError_t gen_inverse32(uint32_t* out, bool f16, int32_t shift, hipStream_t s);
 hipError_t gen_inverse32_f64(double* out, int32_t shift, hipStream_t s);
+// dst[0 .. n) = w[0 .. n) from host memory through kernel arguments (chunks of kTapsPerLaunch), so
+// that no host-to-device copy is enqueued (the allocation-free entry point runs under graph capture).
+constexpr int kTapsPerLaunch = 256;
+hipError_t put_taps(double* dst, const double* w, int64_t n, hipStream_t s);
 // d[i] = output i of SplitMix64(seed) as a double in [0, 1) (the host ffs_fill_uniform sequence)
 hipError_t fill_uniform_device(double* d, int64_t n, uint64_t seed, hipStream_t s);
 // layout 0: FP64 accumulator order (row freq q + 4v); layout 1: FP32 accumulator order (row freq 4q + v)

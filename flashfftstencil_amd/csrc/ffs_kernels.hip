@@ -1869,6 +1869,25 @@ __global__ void k_gen_spectrum(double* H, int kind, int layout, int32_t L0, int3
   H[len + e] = zero ? 0.0 : ki;
 }
 
+struct TapChunk {
+  double w[kTapsPerLaunch];
+  int32_t n;
+};
+
+__global__ void k_put_taps(double* dst, TapChunk c) {
+  for (int i = threadIdx.x; i < c.n; i += blockDim.x) dst[i] = c.w[i];
+}
+
+hipError_t put_taps(double* dst, const double* w, int64_t n, hipStream_t s) {
+  for (int64_t o = 0; o < n; o += kTapsPerLaunch) {
+    TapChunk c{};
+    c.n = (int32_t)std::min<int64_t>(kTapsPerLaunch, n - o);
+    for (int i = 0; i < c.n; i++) c.w[i] = w[o + i];
+    hipLaunchKernelGGL(k_put_taps, dim3(1), dim3(256), 0, s, dst + o, c);
+  }
+  return hipGetLastError();
+}
+
 hipError_t gen_operands(double* ops, uint32_t* opsm, const int32_t* axis_len, hipStream_t s) {
   hipLaunchKernelGGL(k_gen_operands, dim3(3), dim3(256), 0, s, ops, axis_len[0], axis_len[1], axis_len[2]);
   if (opsm)

@@ -191,6 +191,23 @@ ffs_status ffs_run(ffs_plan* plan, const double* d_in, double* d_out, void* stre
 /* Same as ffs_run on host buffers (allocates, copies, runs, synchronises). */
 ffs_status ffs_run_host(ffs_plan* plan, const double* h_in, double* h_out);
 
+/* Allocation-free run: d_out = S^T(d_in) for problem `p` in one call that allocates no device memory.
+ * No counterpart in the reference; it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It plans on the host exactly as
+ * ffs_plan_create does, writes the plan's DFT operands, spectra and taps into a static device array of
+ * FFS_STATIC_SCRATCH_BYTES per GPU (held by every GPU the library runs on, from library load until the
+ * process ends; its address is resolved once per device on the first call there), keeps the ping-pong
+ * grid in the same array, and enqueues the T steps on `stream` as plain kernel launches (no graph, no
+ * host-to-device copy, no hipMalloc/hipFree/hipMemset).  Results are bitwise those of ffs_plan_create +
+ * ffs_run for the same problem.  Limit: the tables take at most FFS_STATIC_TABLE_BYTES, and a run of two
+ * or more fused blocks needs 8 bytes per grid point of the rest (grids up to 8,257,536 points);
+ * larger problems get FFS_ERR_UNSUPPORTED before any device work.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other threads) race, so serialise them
+ * (e.g. one stream).  Nothing in the array needs clearing: each call writes every byte it reads. */
+#define FFS_STATIC_SCRATCH_BYTES (64ull << 20)
+#define FFS_STATIC_TABLE_BYTES (1ull << 20)
+ffs_status ffs_run_static(const ffs_problem* p, const double* d_in, double* d_out, void* stream);
+
 /* One fused block only (depth t, or the tail depth when `tail` != 0 and the plan has a tail),
  * d_out = S^t(d_in).  One kernel launch; used to time the tile kernel alone. */
 ffs_status ffs_run_block(ffs_plan* plan, const double* d_in, double* d_out, int32_t tail, void* stream);
