@@ -416,6 +416,7 @@ ffsk::TileArgs make_args(const ffs_plan* pl, const double* src, double* dst, boo
   a.H = pl->d_H + (tail ? 2 * pl->hlen : 0);
   a.Hf = pl->d_Hf ? pl->d_Hf + (tail ? 2 * pl->hlen : 0) : nullptr;
   a.opsm = pl->d_opsm;
+  a.unit = 1.0f;
   a.dc_gain = tail ? pl->dc_tail : pl->dc_main;
   const ffs_problem& p = pl->prob;
   if (p.dims == 1) {

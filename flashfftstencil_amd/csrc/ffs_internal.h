@@ -69,6 +69,7 @@ struct TileArgs {
   // read those columns from the forward operands they hold in LDS instead of `opsi`).
   int32_t out16;
   const void* opsi;  // out16: inverse operands (kInv32Words words or kInv32F64 doubles, see below)
+  float unit;  // 1.0f: a multiplier the kernels must not see as a constant (the FP16 split's residual FMA)
 };
 
 // Direct (non-FFT) FP64 kernels with temporal blocking (ffs_direct.hip).

@@ -1081,22 +1081,22 @@ __device__ __forceinline__ uint32_t pk_f16(float a, float b) {
   return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{a, b}, h2));
 }
 
-// The FP16 residuals (a - h.lo, b - h.hi) of one word, each rounded once to FP16.  a - h is exact in FP32
-// (h is a's own FP16 rounding), so the only rounding is the final conversion.
-__device__ __forceinline__ uint32_t resid_f16(uint32_t hw, float a, float b) {
-  const h2 h = __builtin_bit_cast(h2, hw);
-  return pk_f16(a - (float)h[0], b - (float)h[1]);
-}
-
-// Same element order as split8: word k of the operand holds elements (2k, 2k+1) of (x0 | x1).
-__device__ __forceinline__ void split8_f16(const f4& x0, const f4& x1, h8& h, h8& l) {
+// Same element order as split8: word k of the operand holds elements (2k, 2k+1) of (x0 | x1); h = f16(x),
+// l = f16(x - h).  x - h is exact in FP32 (h is x's own FP16 rounding), so l is x's residual rounded once.
+// The residual is written as fma(x, unit, -h) with `unit` = 1 held in a register: the compiler then folds
+// the FP16 -> FP32 conversion of h into a v_fma_mix (with a literal 1 it would fold the FMA into a
+// subtraction and convert h separately), 4 instructions per word instead of 6.  h is bit-cast from the
+// scalar word, never from the vector element hw[k] (a bit_cast of that element read element 0 here).
+__device__ __forceinline__ void split8_f16(const f4& x0, const f4& x1, float unit, h8& h, h8& l) {
   u4 hw, lw;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const f4& x = k < 2 ? x0 : x1;
     const int e = 2 * (k & 1);
-    hw[k] = pk_f16(x[e], x[e + 1]);
-    lw[k] = resid_f16(hw[k], x[e], x[e + 1]);
+    const uint32_t hk = pk_f16(x[e], x[e + 1]);
+    const h2 hh = __builtin_bit_cast(h2, hk);
+    hw[k] = hk;
+    lw[k] = pk_f16(__builtin_fmaf(x[e], unit, -(float)hh[0]), __builtin_fmaf(x[e + 1], unit, -(float)hh[1]));
   }
   h = as_h8(hw);
   l = as_h8(lw);
@@ -1147,45 +1147,53 @@ __device__ __forceinline__ float wave_max_nonneg(float v) {
 }
 
 // SCALED: the inputs are split as z * sc (the first forward stage, which applies the tile's scale).
+// All NI input row blocks are split first (the inputs are dead afterwards, so the split operands take
+// their registers), then each column block Kb of the operands is read from LDS once and used for the NI
+// output blocks (I, Kb): half the operand reads of splitting and loading per (I, Kb), and four
+// independent accumulator chains per operand load.  Every output block sees the same MFMAs in the same
+// order as when computed alone.
 template <bool INV, int NI = 2, int NK = 2, int KBS = 2, bool SCALED = false>
 __device__ __forceinline__ void stage32_f16(const Blk4& zr, const Blk4& zi, const u4* __restrict__ sF, int lane,
-                                            Blk4& outr, Blk4& outi, float sc = 1.0f) {
+                                            Blk4& outr, Blk4& outi, float sc, float unit) {
+  h8 rh[NI], rl[NI], ih[NI], il[NI];
 #pragma unroll
   for (int I = 0; I < NI; I++) {
-    h8 rh, rl, ih, il;
     const f4 x0 = zr.b[I], x1 = zr.b[2 + I], y0 = zi.b[I], y1 = zi.b[2 + I];
     if (SCALED) {
-      split8_f16_scaled(x0, x1, sc, rh, rl);
-      split8_f16_scaled(y0, y1, sc, ih, il);
+      split8_f16_scaled(x0, x1, sc, rh[I], rl[I]);
+      split8_f16_scaled(y0, y1, sc, ih[I], il[I]);
     } else {
-      split8_f16(x0, x1, rh, rl);
-      split8_f16(y0, y1, ih, il);
+      split8_f16(x0, x1, unit, rh[I], rl[I]);
+      split8_f16(y0, y1, unit, ih[I], il[I]);
     }
+  }
 #pragma unroll
-    for (int Kb = 0; Kb < NK; Kb++) {
-      h8 Fr[2], Fi[2], Fn[2];
+  for (int Kb = 0; Kb < NK; Kb++) {
+    h8 Fr[2], Fi[2], Fn[2];
 #pragma unroll
-      for (int t = 0; t < 2; t++) {
-        Fr[t] = as_h8(sF[((0 * KBS + Kb) * 3 + t) * kLanes + lane]);
-        Fi[t] = as_h8(sF[((1 * KBS + Kb) * 3 + t) * kLanes + lane]);
-        Fn[t] = as_h8(sF[((2 * KBS + Kb) * 3 + t) * kLanes + lane]);
-      }
-      // forward: Re = zr Fr + zi (-Fi), Im = zr Fi + zi Fr;  inverse: Re = zr Fr + zi Fi, Im = zr (-Fi) + zi Fr
-      const h8* Gi = INV ? Fn : Fi;  // multiplies zr into Im
-      const h8* Gr = INV ? Fi : Fn;  // multiplies zi into Re
+    for (int t = 0; t < 2; t++) {
+      Fr[t] = as_h8(sF[((0 * KBS + Kb) * 3 + t) * kLanes + lane]);
+      Fi[t] = as_h8(sF[((1 * KBS + Kb) * 3 + t) * kLanes + lane]);
+      Fn[t] = as_h8(sF[((2 * KBS + Kb) * 3 + t) * kLanes + lane]);
+    }
+    // forward: Re = zr Fr + zi (-Fi), Im = zr Fi + zi Fr;  inverse: Re = zr Fr + zi Fi, Im = zr (-Fi) + zi Fr
+    const h8* Gi = INV ? Fn : Fi;  // multiplies zr into Im
+    const h8* Gr = INV ? Fi : Fn;  // multiplies zi into Re
+#pragma unroll
+    for (int I = 0; I < NI; I++) {
       f4 re = {0.f, 0.f, 0.f, 0.f}, iv = re;
-      re = FFS_MFMA_F16(rh, Fr[1], re);  // the cross products first, the two dominant ones last
-      re = FFS_MFMA_F16(rl, Fr[0], re);
-      iv = FFS_MFMA_F16(rh, Gi[1], iv);
-      iv = FFS_MFMA_F16(rl, Gi[0], iv);
-      re = FFS_MFMA_F16(ih, Gr[1], re);
-      re = FFS_MFMA_F16(il, Gr[0], re);
-      iv = FFS_MFMA_F16(ih, Fr[1], iv);
-      iv = FFS_MFMA_F16(il, Fr[0], iv);
-      re = FFS_MFMA_F16(rh, Fr[0], re);
-      iv = FFS_MFMA_F16(rh, Gi[0], iv);
-      re = FFS_MFMA_F16(ih, Gr[0], re);
-      iv = FFS_MFMA_F16(ih, Fr[0], iv);
+      re = FFS_MFMA_F16(rh[I], Fr[1], re);  // the cross products first, the two dominant ones last
+      re = FFS_MFMA_F16(rl[I], Fr[0], re);
+      iv = FFS_MFMA_F16(rh[I], Gi[1], iv);
+      iv = FFS_MFMA_F16(rl[I], Gi[0], iv);
+      re = FFS_MFMA_F16(ih[I], Gr[1], re);
+      re = FFS_MFMA_F16(il[I], Gr[0], re);
+      iv = FFS_MFMA_F16(ih[I], Fr[1], iv);
+      iv = FFS_MFMA_F16(il[I], Fr[0], iv);
+      re = FFS_MFMA_F16(rh[I], Fr[0], re);
+      iv = FFS_MFMA_F16(rh[I], Gi[0], iv);
+      re = FFS_MFMA_F16(ih[I], Gr[0], re);
+      iv = FFS_MFMA_F16(ih[I], Fr[0], iv);
       outr.b[2 * I + Kb] = re;
       outi.b[2 * I + Kb] = iv;
     }
@@ -1214,8 +1222,8 @@ __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, c
   int k;
   (void)frexpf(m, &k);  // m < 2^k
   const int e = __builtin_amdgcn_readfirstlane(min(max(a.scale_exp - k, -120), 120));
-  stage32_f16<false, 2, 2, 2, true>(zr, zi, sF, lane, orr, oi, ldexpf(1.0f, e));
-  stage32_f16<false>(orr, oi, sF, lane, zr, zi);
+  stage32_f16<false, 2, 2, 2, true>(zr, zi, sF, lane, orr, oi, ldexpf(1.0f, e), a.unit);
+  stage32_f16<false>(orr, oi, sF, lane, zr, zi, 1.0f, a.unit);
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int b = 0; b < 4; b++)
@@ -1231,12 +1239,12 @@ __device__ __forceinline__ void tile32_pair_f16(const TileArgs& a, int64_t tp, c
     }
   if (OUT16) {
     const int il = ilane < 0 ? lane : ilane;  // the lane whose inverse operands this lane reads
-    stage32_f16<true, 2, 1, IKBS>(zr, zi, sG, il, orr, oi);
-    stage32_f16<true, 1, 1, IKBS>(orr, oi, sG, il, zr, zi);
+    stage32_f16<true, 2, 1, IKBS>(zr, zi, sG, il, orr, oi, 1.0f, a.unit);
+    stage32_f16<true, 1, 1, IKBS>(orr, oi, sG, il, zr, zi, 1.0f, a.unit);
     store16(a, pc, q, j, zr.b[0], zi.b[0], cA * a.dc_gain, cB * a.dc_gain, ldexp(1.0, -e));
   } else {
-    stage32_f16<true>(zr, zi, sF, lane, orr, oi);
-    stage32_f16<true>(orr, oi, sF, lane, zr, zi);
+    stage32_f16<true>(zr, zi, sF, lane, orr, oi, 1.0f, a.unit);
+    stage32_f16<true>(orr, oi, sF, lane, zr, zi, 1.0f, a.unit);
     store32(a, pc, q, j, zr, zi, cA * a.dc_gain, cB * a.dc_gain, ldexp(1.0, -e));
   }
 }
