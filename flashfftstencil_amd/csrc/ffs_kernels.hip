@@ -510,6 +510,22 @@ __global__ __launch_bounds__(256, 2) void k_tile32_f64(TileArgs a) {
 
 // Multi-grid launch of the FP64 form (the k_batch32_* structure): the DFT operands, the same for every
 // item, in LDS; each item's spectrum read per tile pair from its FP64 table (L2-resident).
+// Work item idx of a batch round -> grid g with prefix[g] <= idx < prefix[g + 1].  A wave takes its items in
+// increasing order, so the search starts after the grid of its previous item and most items stay in that
+// grid (no dependent scalar loads); a change costs one binary search over the later grids.  g = -1 and
+// next = 0 before the first item.
+__device__ __forceinline__ int batch_grid(const int64_t* __restrict__ prefix, int32_t n, int64_t idx, int g,
+                                          int64_t& next) {
+  if (idx < next) return g;
+  int lo = g + 1, hi = n;  // prefix[lo] = next <= idx
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= idx) lo = mid; else hi = mid;
+  }
+  next = prefix[lo + 1];
+  return lo;
+}
+
 __global__ __launch_bounds__(256, 2) void k_batch32_f64(const TileArgs* __restrict__ items,
                                                         const int64_t* __restrict__ prefix, int32_t n) {
   __shared__ double sF[kOps32F64];
@@ -520,13 +536,11 @@ __global__ __launch_bounds__(256, 2) void k_batch32_f64(const TileArgs* __restri
   const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t gw = block_order(items[0].linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t total = prefix[n];
+  int lo = -1;
+  int64_t next = 0;
   for (int64_t idx = gw; idx < total; idx += W) {
-    int lo = 0, hi = n;  // largest g with prefix[g] <= idx
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (prefix[mid] <= idx) lo = mid; else hi = mid;
-    }
-    const TileArgs a = items[lo];
+    lo = batch_grid(prefix, n, idx, lo, next);
+    const TileArgs a = items[lo];  // independent scalar loads, from the scalar cache while lo is unchanged
     if (a.out16) {
       const ShiftedView<double> v = shifted_view(sF, a.h1, lane);
       tile32_pair_f64<false, true, 2>(a, idx - prefix[lo], sF, nullptr, v.base, lane, q, j, v.lane);
@@ -1300,13 +1314,11 @@ __global__ __launch_bounds__(256, 3) void k_batch32_bf16(const TileArgs* __restr
   const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t gw = block_order(items[0].linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t total = prefix[n];
+  int lo = -1;
+  int64_t next = 0;
   for (int64_t idx = gw; idx < total; idx += W) {
-    int lo = 0, hi = n;  // largest g with prefix[g] <= idx
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (prefix[mid] <= idx) lo = mid; else hi = mid;
-    }
-    const TileArgs a = items[lo];
+    lo = batch_grid(prefix, n, idx, lo, next);
+    const TileArgs a = items[lo];  // independent scalar loads, from the scalar cache while lo is unchanged
     if (a.out16) {
       const ShiftedView<u4> v = shifted_view(sF, a.h1, lane);
       tile32_pair<NTERMS, false, true, 2>(a, idx - prefix[lo], sF, nullptr, v.base, lane, q, j, v.lane);
@@ -1344,13 +1356,11 @@ __global__ __launch_bounds__(256, 3) void k_batch32_f16(const TileArgs* __restri
   const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t gw = block_order(items[0].linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t total = prefix[n];
+  int lo = -1;
+  int64_t next = 0;
   for (int64_t idx = gw; idx < total; idx += W) {
-    int lo = 0, hi = n;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (prefix[mid] <= idx) lo = mid; else hi = mid;
-    }
-    const TileArgs a = items[lo];
+    lo = batch_grid(prefix, n, idx, lo, next);
+    const TileArgs a = items[lo];  // independent scalar loads, from the scalar cache while lo is unchanged
     if (a.out16) {
       const ShiftedView<u4> v = shifted_view(sF, a.h1, lane);
       tile32_pair_f16<false, true, 2>(a, idx - prefix[lo], sF, nullptr, v.base, lane, q, j, v.lane);
