@@ -1,0 +1,161 @@
+"""Parity at the benchmark workloads' full sizes (8192^2, 512^3, 2^20 points), where a full oracle sweep
+would take minutes, through checks that do not depend on the size:
+
+  * sampled points of the exact T-step operator: by linearity and shift invariance, T steps of the
+    periodic stencil are one convolution with the T-fold kernel K_T, out[p] = sum_a K_T[a] in[p - a]
+    (ref convention).  K_T itself comes from the oracle (a delta swept T steps on a grid of side 2rT + 1,
+    where nothing wraps onto itself), and out[p] is evaluated in FP64 at a few thousand points, including
+    the first and last rows/columns/planes where the periodic wrap is exercised;
+  * a Fourier mode: cos(2 pi k.x / n + phi) is an eigenvector of every periodic stencil, so T steps give
+    |K(k)|^T cos(2 pi k.x / n + phi + T arg K(k)) at every point, with K(k) the taps' DFT at k;
+  * mass: sum(out) = sum(in) (sum w)^T for any weights.
+Bars: FP64 1e-12, mixed forms 1e-6 (max error over the checked points / max |reference| there).  The
+workloads: Box2D25P and Box2D9P 8192^2 (SplitMix64 uniform grids and weights), Box3D27P 512^3 (the
+seeded sinusoid field) and Heat-1D 2^20 (checked against a full oracle sweep, which is cheap in 1D).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BARS = {"fp64": 1e-12, "f16": 1e-6, "bf16": 1e-6}
+
+
+def t_fold_kernel(oracle, dims, radius, w, steps):
+    """K_T on [0, 2 r T]^dims: the oracle's T-step response to a delta at the origin (ref convention)."""
+    side = 2 * radius * steps + 1
+    delta = np.zeros((side,) * dims)
+    delta[(0,) * dims] = 1.0
+    return oracle.sweep(delta, radius, w, steps)
+
+
+def sample_points(shape, count, seed):
+    """Random points plus every point of the first and last index along each axis' edge lines."""
+    rng = np.random.default_rng(seed)
+    pts = [rng.integers(0, n, size=count) for n in shape]
+    edges = []
+    for ax, n in enumerate(shape):
+        for e in (0, n - 1):
+            k = 64
+            cols = [rng.integers(0, m, size=k) for m in shape]
+            cols[ax] = np.full(k, e)
+            edges.append(cols)
+    return [np.concatenate([pts[a]] + [c[a] for c in edges]) for a in range(len(shape))]
+
+
+def apply_at(x, kt, points):
+    """sum_a kt[a] x[(p - a) mod n] at every sample point p (FP64)."""
+    offs = np.stack(np.meshgrid(*[np.arange(s) for s in kt.shape], indexing="ij"), -1).reshape(-1, kt.ndim)
+    kflat = kt.reshape(-1)
+    nz = kflat != 0
+    offs, kflat = offs[nz], kflat[nz]
+    out = np.zeros(points[0].size)
+    for a, kv in zip(offs, kflat):
+        idx = tuple((points[d] - a[d]) % x.shape[d] for d in range(x.ndim))
+        out += kv * x[idx]
+    return out
+
+
+def run_plan(ffs, torch, x, radius, w, steps, form):
+    d_in = torch.from_numpy(x).cuda()
+    d_out = torch.empty_like(d_in)
+    with ffs.Plan(x.ndim, x.shape, w, radius, steps, precision=form) as plan:
+        plan.run(d_in, d_out)
+        info = plan.info
+    torch.cuda.synchronize()
+    del d_in
+    return d_out, info
+
+
+_FIELDS = {}
+
+
+def sinusoid_field(ffs, seed, sizes):
+    """The 3D workload's field: 8 sinusoids, amplitude uniform[0,1)/8, phase uniform[0, 2 pi), integer
+    wavevectors uniform in [1, 16]^3, plus 1e-3 uniform noise; all draws from SplitMix64(seed)."""
+    if (seed, sizes) in _FIELDS:
+        return _FIELDS[(seed, sizes)]
+    u = ffs.fill_uniform(seed, 8 * 5)
+    n0, n1, n2 = sizes
+    x = np.zeros(sizes)
+    i0 = np.arange(n0)[:, None, None] / n0
+    i1 = np.arange(n1)[None, :, None] / n1
+    i2 = np.arange(n2)[None, None, :] / n2
+    for m in range(8):
+        amp, ph = u[5 * m] / 8, 2 * np.pi * u[5 * m + 1]
+        k = [1 + int(16 * u[5 * m + 2 + a]) for a in range(3)]
+        x += amp * np.cos(2 * np.pi * (k[0] * i0 + k[1] * i1 + k[2] * i2) + ph)
+    x += 1e-3 * ffs.fill_uniform(seed + 1, x.size).reshape(sizes)
+    _FIELDS[(seed, sizes)] = x
+    return x
+
+
+@pytest.mark.parametrize("shape,steps,form", [
+    ("Box2D25P", 12, "f16"), ("Box2D25P", 12, "bf16"), ("Box2D25P", 9, "fp64"),
+    ("Box2D9P", 17, "fp64"), ("Box2D9P", 17, "f16"),
+])
+def test_8192_sampled_operator(ffs, oracle, torch_cuda, shape, steps, form):
+    seed = 0x5EED0002 if shape == "Box2D25P" else 0x5EED0001
+    _, r, w, _ = ffs.named_stencil(shape, seed=seed)
+    n = 8192
+    x = ffs.fill_uniform(seed + 1, n * n).reshape(n, n)
+    d_out, info = run_plan(ffs, torch_cuda, x, r, w, steps, form)
+    assert info["blocks"] >= 2, info  # main blocks and (for these T) a tail
+    pts = sample_points((n, n), 4096, steps)
+    ref = apply_at(x, t_fold_kernel(oracle, 2, r, w, steps), pts)
+    got = d_out.cpu().numpy()[tuple(pts)]
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err <= BARS[form], (shape, form, steps, info, err)
+    # mass: sum(out) = sum(in) (sum w)^T, in FP64 on the device
+    mass = float(d_out.sum())
+    want = float(x.sum()) * float(np.sum(w)) ** steps
+    assert abs(mass - want) <= (1e-12 if form == "fp64" else 1e-6) * abs(want), (mass, want)
+
+
+@pytest.mark.parametrize("form", ["f16", "fp64"])
+def test_8192_fourier_mode(ffs, torch_cuda, form):
+    # an eigenmode of the periodic Box2D25P stencil at 8192^2: the exact answer at every point
+    _, r, w, _ = ffs.named_stencil("Box2D25P", seed=0x5EED0002)
+    n, steps = 8192, 40
+    k0, k1, phi = 37, 90, 0.3  # a slowly decaying mode: |K(k)|^T stays near 1, its phase advances
+    i0 = np.arange(n)[:, None]
+    i1 = np.arange(n)[None, :]
+    theta = 2 * np.pi * (((k0 * i0) % n) / n + ((k1 * i1) % n) / n)
+    x = 0.5 + np.cos(theta + phi)  # plus a constant (the DC eigenvector, gain (sum w)^T = 1)
+    K = 0j
+    for a0 in range(2 * r + 1):
+        for a1 in range(2 * r + 1):
+            K += w[a0 * (2 * r + 1) + a1] * np.exp(-2j * np.pi * (((a0 * k0) % n) / n + ((a1 * k1) % n) / n))
+    KT = K ** steps
+    want = 0.5 * np.sum(w) ** steps + np.abs(KT) * np.cos(theta + phi + np.angle(KT))
+    d_out, info = run_plan(ffs, torch_cuda, x, r, w, steps, form)
+    err = np.abs(d_out.cpu().numpy() - want).max() / np.abs(want).max()
+    assert err <= BARS[form], (form, info, err)
+
+
+@pytest.mark.parametrize("form", ["bf16", "fp64"])
+def test_512cube_sampled_operator(ffs, oracle, torch_cuda, form):
+    # the 3D workload's sinusoid field at 512^3, T = 4 (two blocks of depth 2)
+    _, r, w, _ = ffs.named_stencil("Box3D27P", seed=0x5EED0003)
+    sizes, steps = (512, 512, 512), 4
+    x = sinusoid_field(ffs, 0x5EED0004, sizes)
+    d_out, info = run_plan(ffs, torch_cuda, x, r, w, steps, form)
+    pts = sample_points(sizes, 4096, 7)
+    ref = apply_at(x, t_fold_kernel(oracle, 3, r, w, steps), pts)
+    got = d_out.cpu().numpy()[tuple(pts)]
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err <= BARS[form], (form, info, err)
+
+
+@pytest.mark.parametrize("form,algorithm", [("fp64", "fft"), ("fp64", "direct"), ("bf16", "fft")])
+def test_heat1d_2p20_full_oracle(ffs, oracle, torch_cuda, form, algorithm):
+    n, steps = 1 << 20, 64
+    _, r, w, _ = ffs.named_stencil("Heat-1D")
+    x = ffs.fill_uniform(0x5EED0001, n)
+    d_in = torch_cuda.from_numpy(x).cuda()
+    d_out = torch_cuda.empty_like(d_in)
+    with ffs.Plan(1, (n,), w, r, steps, precision=form, algorithm=algorithm) as plan:
+        plan.run(d_in, d_out)
+    torch_cuda.cuda.synchronize()
+    err = oracle.errors(d_out.cpu().numpy(), oracle.sweep(x, r, w, steps))[1]
+    assert err <= BARS[form], (form, algorithm, err)

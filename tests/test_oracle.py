@@ -167,3 +167,20 @@ def test_splitmix_uniform_known_values():
     u = O.uniform(0, (3,))
     first = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
     assert list(u) == [(v >> 11) * 2.0 ** -53 for v in first]
+
+
+@pytest.mark.parametrize("dims,shape,radius", [(1, (301,), 3), (2, (67, 53), 2), (3, (17, 19, 13), 1)])
+def test_t_fold_operator_matches_sweep(oracle, dims, shape, radius):
+    # the full-size GPU tests evaluate T steps at sample points as one convolution with the T-fold kernel
+    # (the oracle's response to a delta); on small grids that must equal the oracle's own sweep there,
+    # including the wrap-around rows and columns
+    from tests.test_gpu_fullsize import apply_at, sample_points, t_fold_kernel
+    rng = np.random.default_rng(dims)
+    w = rng.random((2 * radius + 1) ** dims)
+    w /= w.sum()
+    x = oracle.uniform(5 + dims, shape)
+    for steps in (1, 3, 5):
+        pts = sample_points(shape, 300, steps)
+        got = apply_at(x, t_fold_kernel(oracle, dims, radius, w, steps), pts)
+        ref = oracle.sweep(x, radius, w, steps)[tuple(pts)]
+        assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
