@@ -25,6 +25,8 @@ def lib():
         L.orc_sweep.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.c_int, D, ctypes.c_int,
                                 ctypes.c_int64, D, D, ctypes.c_int]
         L.orc_sweep.restype = ctypes.c_int
+        L.orc_sweep_fast.argtypes = L.orc_sweep.argtypes
+        L.orc_sweep_fast.restype = ctypes.c_int
         L.orc_fill_uniform.argtypes = [ctypes.c_uint64, D, ctypes.c_int64]
         L.orc_fill_uniform.restype = None
         L.orc_num_threads.restype = ctypes.c_int
@@ -44,6 +46,18 @@ def sweep(x, radius, weights, steps, convention=0, threads=0):
     out = np.empty_like(x)
     sizes = (ctypes.c_int64 * 3)(*(list(x.shape) + [1] * (3 - x.ndim)))
     rc = lib().orc_sweep(x.ndim, sizes, radius, _p(w), convention, steps, _p(x), _p(out), threads)
+    assert rc == 0
+    return out
+
+
+def sweep_fast(x, radius, weights, steps, convention=0, threads=0):
+    """The same sweep evaluated for speed (oracle/sweep_fast.c: register-blocked FMAs); the CPU baseline."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).ravel())
+    assert w.size == (2 * radius + 1) ** x.ndim
+    out = np.empty_like(x)
+    sizes = (ctypes.c_int64 * 3)(*(list(x.shape) + [1] * (3 - x.ndim)))
+    rc = lib().orc_sweep_fast(x.ndim, sizes, radius, _p(w), convention, steps, _p(x), _p(out), threads)
     assert rc == 0
     return out
 

@@ -184,3 +184,31 @@ def test_t_fold_operator_matches_sweep(oracle, dims, shape, radius):
         got = apply_at(x, t_fold_kernel(oracle, dims, radius, w, steps), pts)
         ref = oracle.sweep(x, radius, w, steps)[tuple(pts)]
         assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dims,shapes", [
+    (1, [(1,), (3,), (7,), (31,), (33,), (4099,), (70001,)]),
+    (2, [(1, 1), (2, 5), (5, 3), (17, 40), (64, 97), (33, 1100)]),
+    (3, [(1, 2, 3), (4, 5, 6), (9, 8, 37), (6, 7, 70)]),
+])
+def test_fast_sweep_matches_checker(dims, shapes):
+    """oracle/sweep_fast.c (the timed CPU baseline: register blocks, overlapping tail block, cheap wraps,
+    column strips) against the checker sweep, over rows shorter and longer than its 32-point blocks and
+    512-point strips, grids smaller than the stencil, radii 1..3, zero taps and both conventions."""
+    rng = np.random.default_rng(dims)
+    for shape in shapes:
+        for radius in (1, 2, 3):
+            for conv in (0, 1):
+                x = rng.random(shape) - 0.25
+                w = rng.random((2 * radius + 1) ** dims) - 0.3
+                w[rng.random(w.size) < 0.2] = 0.0
+                a = O.sweep(x, radius, w, 6, conv)
+                b = O.sweep_fast(x, radius, w, 6, conv)
+                err = np.abs(a - b).max() / np.abs(a).max()
+                assert err <= 1e-13, (shape, radius, conv, err)
+    # zero steps copy the grid; a repeated call gives the same result
+    x = rng.random(shapes[-1])
+    w = rng.random(3 ** dims)
+    ref = O.sweep(x, 1, w, 3)
+    assert np.abs(O.sweep_fast(x, 1, w, 0) - x).max() == 0.0
+    assert np.abs(O.sweep_fast(x, 1, w, 3) - ref).max() <= 1e-13 * np.abs(ref).max()

@@ -1,7 +1,7 @@
 """Host code under AddressSanitizer + UndefinedBehaviorSanitizer (CPU only, no GPU needed).
 
 Three builds, each from the committed sources into a temporary directory:
-  * the oracle (oracle/oracle.c) with gcc, driven by tests/sanitize/oracle_check.c over 1D/2D/3D ragged
+  * the oracle (oracle/oracle.c and oracle/sweep_fast.c) with gcc, driven by tests/sanitize/oracle_check.c over 1D/2D/3D ragged
     and tiny grids, both conventions, T = 0, aliasing and rejected arguments;
   * libffstencil with the sanitizers on its host code only (device code cannot take them: on every hipcc
     line each -fsanitize= follows -Xarch_host), driven by tests/sanitize/host_abi_check.cpp through every
@@ -67,9 +67,9 @@ def test_oracle_under_sanitizers(san_dir):
     _need("gcc")
     exe = str(san_dir / "oracle_check")
     r = _run(["gcc", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
-              "-fno-sanitize-recover=undefined", "-fopenmp", "-std=c11", "-Wall",
+              "-fno-sanitize-recover=undefined", "-fopenmp", "-std=c11", "-Wall", "-march=x86-64-v3",
               os.path.join(ROOT, "tests", "sanitize", "oracle_check.c"), os.path.join(ROOT, "oracle", "oracle.c"),
-              "-lm", "-o", exe])
+              os.path.join(ROOT, "oracle", "sweep_fast.c"), "-lm", "-o", exe])
     assert r.returncode == 0, r.stderr
     r = _run([exe], env=SAN_ENV)
     _clean(r)
