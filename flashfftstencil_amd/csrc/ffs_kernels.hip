@@ -1493,11 +1493,18 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
   typedef typename P::V4 V4;
   typedef typename P::C2 C2;
   __shared__ C2 sX[16 * 16 * 17];
+  // Mixed form: planes k2 = 0..8 of the FP32 spectrum in LDS (18 KB; with the window 52 KB, 3 workgroups
+  // per CU).  Real taps make the spectrum Hermitian, H(-k mod 16) = conj H(k), so planes 9..15 are read
+  // as conjugates of planes 7..1: the same FP32 values as the global table, without its L2 latency in
+  // phase 2.  First read in phase 2, after the barriers of the window fill.
+  __shared__ f2 sH[PREC != 0 ? 9 * 4 * kLanes : 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int q = lane >> 4, j = lane & 15;
   typename P::Ops f;
   P::load_ops(f, a, lane);
   const bool mixed = PREC != 0;
+  if (mixed)
+    for (int i = threadIdx.x; i < 9 * 4 * kLanes; i += blockDim.x) sH[i] = reinterpret_cast<const f2*>(a.Hf)[i];
   // Phase loops: fully unrolled in the mixed form (later slices' loads issue early, 1.5% faster), two
   // slices at a time in the FP64 form (full unrolling costs it 3% at its 2 waves per SIMD).
   constexpr int kSlices = PREC == 0 ? 2 : 4;
@@ -1580,14 +1587,20 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
       P::template stage<false>(pr[s], pi[s], f, br, bi);  // [k1 = row][k0 = lane]
       // H entry (k2, v, lane): k0 = j, k1 = rowmap(q, v), k2
       const double* H = a.H + (int64_t)k2 * 4 * kLanes + lane;
-      const f2* Hf = reinterpret_cast<const f2*>(a.Hf) + (int64_t)k2 * 4 * kLanes + lane;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         V hr, hi;
-        if (mixed) {  // the FP32 copy: one 8-byte load instead of two FP64 loads and conversions
-          const f2 hv = Hf[v * kLanes];
-          hr = (V)hv[0];
-          hi = (V)hv[1];
+        if (mixed) {  // the FP32 copy, from LDS; k2 is wave-uniform, so is the branch
+          if (k2 <= 8) {
+            const f2 hv = sH[k2 * 4 * kLanes + v * kLanes + lane];
+            hr = (V)hv[0];
+            hi = (V)hv[1];
+          } else {  // conj H at (-k0, -k1, 16 - k2) mod 16; k1 = 4q + v in the FP32 accumulator order
+            const int k1c = (16 - rowmap<PREC>(q, v)) & 15, k0c = (16 - j) & 15;
+            const f2 hv = sH[(16 - k2) * 4 * kLanes + (k1c & 3) * kLanes + 16 * (k1c >> 2) + k0c];
+            hr = (V)hv[0];
+            hi = -(V)hv[1];
+          }
         } else {
           hr = (V)H[v * kLanes];
           hi = (V)H[4096 + v * kLanes];
