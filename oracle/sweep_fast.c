@@ -47,7 +47,7 @@ static inline int64_t min64(int64_t a, int64_t b) { return a < b ? a : b; }
 static inline int64_t max64(int64_t a, int64_t b) { return a > b ? a : b; }
 
 /* Outputs o[i2 .. i2 + 32) of an interior stretch: 8 AVX2 accumulators, one FMA per tap each. */
-static inline void block32(const double* const* src, const Tap* taps, int ntap, int64_t i2, double* o) {
+static inline void block32(const double** src, const Tap* taps, int ntap, int64_t i2, double* o) {
   __m256d a0 = _mm256_setzero_pd(), a1 = a0, a2 = a0, a3 = a0, a4 = a0, a5 = a0, a6 = a0, a7 = a0;
   for (int t = 0; t < ntap; t++) {
     const double* s = src[taps[t].row] + (i2 - taps[t].d);
@@ -72,17 +72,22 @@ static inline void block32(const double* const* src, const Tap* taps, int ntap, 
   _mm256_storeu_pd(d + 28, a7);
 }
 
-static void step_fast(const double* in, double* out, const int64_t n[3], const int kw[3], const int64_t off0[3],
-                      const Tap* taps, int ntap, int64_t dlo, int64_t dhi) {
+/* One step; returns -1 if a thread could not allocate its source-row pointers ((2r+1)^2 of them, on the
+ * heap: a large radius would not fit a thread's stack). */
+static int step_fast(const double* in, double* out, const int64_t n[3], const int kw[3], const int64_t off0[3],
+                     const Tap* taps, int ntap, int64_t dlo, int64_t dhi) {
   const int64_t n0 = n[0], n1 = n[1], n2 = n[2];
   const int64_t nch = (n2 + kChunk - 1) / kChunk;
   const int64_t items = n0 * n1 * nch;
   const int nrows = kw[0] * kw[1];
-#pragma omp parallel
+  int failed = 0;
+#pragma omp parallel reduction(| : failed)
   {
-    const double* src[nrows];
+    const double** src = (const double**)malloc(sizeof(const double*) * (size_t)nrows);
+    if (!src) failed = 1;
 #pragma omp for schedule(static)
     for (int64_t it = 0; it < items; it++) {
+      if (!src) continue;
       /* column strips of kChunk points, each walked down the rows: the source row segments of one
        * output row are mostly those of the previous one, still in L1 */
       const int64_t c = it / (n0 * n1), row = it - c * (n0 * n1);
@@ -123,7 +128,9 @@ static void step_fast(const double* in, double* out, const int64_t n[3], const i
         o[i2] = acc;
       }
     }
+    free(src);
   }
+  return failed ? -1 : 0;
 }
 
 /*
@@ -185,13 +192,14 @@ int orc_sweep_fast(int dims, const int64_t* sizes, int radius, const double* wei
     memcpy(keep, in, sizeof(double) * (size_t)total);
     src = keep;
   }
-  for (int64_t s = 0; s < steps; s++) {
+  int rc = 0;
+  for (int64_t s = 0; s < steps && rc == 0; s++) {
     double* dst = ((steps - 1 - s) % 2 == 0) ? out : tmp;
-    step_fast(src, dst, n, kw, off0, taps, nt, dlo, dhi);
+    rc = step_fast(src, dst, n, kw, off0, taps, nt, dlo, dhi);
     src = dst;
   }
   free(tmp);
   free(keep);
   free(taps);
-  return 0;
+  return rc;
 }

@@ -212,3 +212,13 @@ def test_fast_sweep_matches_checker(dims, shapes):
     ref = O.sweep(x, 1, w, 3)
     assert np.abs(O.sweep_fast(x, 1, w, 0) - x).max() == 0.0
     assert np.abs(O.sweep_fast(x, 1, w, 3) - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_fast_sweep_wide_stencil():
+    """A radius much larger than the grid (every read wraps, (2r+1)^2 source rows per output row)."""
+    rng = np.random.default_rng(7)
+    x = rng.random((23, 41))
+    w = rng.random(121 * 121) - 0.5
+    a = O.sweep(x, 60, w, 2)
+    b = O.sweep_fast(x, 60, w, 2)
+    assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
