@@ -60,6 +60,11 @@ __device__ __forceinline__ void cmul(float& yr, float& yi, float hr, float hi) {
   yi = i;
 }
 
+// Output stores of the tile kernels are non-temporal: a block's output grid is read only by the next
+// block (hundreds of MiB, beyond L2 and MALL), so its lines should not displace the input windows, whose
+// halos the neighbouring pairs re-read (FP16 Box2D25P block +1.5%, 96-grid FP16 batch +2%; same values).
+__device__ __forceinline__ void put_out(double* p, double v) { __builtin_nontemporal_store(v, p); }
+
 // XCD-aware workgroup order: the hardware hands workgroup b to XCD (b mod 8); renumber so that each
 // XCD receives one contiguous range of logical workgroups (neighbouring tiles share its L2).
 __device__ __forceinline__ int64_t xcd_logical_block(int64_t b, int64_t G) {
@@ -209,8 +214,8 @@ __device__ __forceinline__ void store_pair(const TileArgs& a, const PairCoord& p
       const int row = q + 4 * v;
       if (row >= a.h0 && row < a.h0 + a.V0 && pc.r0 + row - a.h0 < a.n0) {
         double* drow = base + (int64_t)(4 * v) * a.n1;
-        if (okA) drow[cA] = yr[v];
-        if (okB) drow[cB] = yi[v];
+        if (okA) put_out(drow + cA, yr[v]);
+        if (okB) put_out(drow + cB, yi[v]);
       }
     }
   } else {
@@ -221,8 +226,8 @@ __device__ __forceinline__ void store_pair(const TileArgs& a, const PairCoord& p
         const int n = (225 * j + 16 * row) % 240;
         if (n >= a.h1 && n < a.h1 + a.V1) {
           const int64_t pA = pc.orgA + n - a.h1, pB = pc.orgB + n - a.h1;
-          if (pA < a.n1) a.dst[pA] = yr[v];
-          if (pc.hasB && pB < a.n1) a.dst[pB] = yi[v];
+          if (pA < a.n1) put_out(a.dst + pA, yr[v]);
+          if (pc.hasB && pB < a.n1) put_out(a.dst + pB, yi[v]);
         }
       }
     }
@@ -412,8 +417,8 @@ __device__ __forceinline__ void store32_f64(const TileArgs& a, const PairCoord& 
         const int64_t y = pc.r0 + row - a.h0;
         if (row >= a.h0 && row < a.h0 + a.V0 && y < a.n0) {
           double* drow = a.dst + y * a.n1;
-          if (okA) drow[cA] = yr.b[2 * R + C][v];
-          if (okB) drow[cB] = yi.b[2 * R + C][v];
+          if (okA) put_out(drow + cA, yr.b[2 * R + C][v]);
+          if (okB) put_out(drow + cB, yi.b[2 * R + C][v]);
         }
       }
   }
@@ -432,8 +437,8 @@ __device__ __forceinline__ void store16_f64(const TileArgs& a, const PairCoord& 
     const int64_t y = pc.r0 + row;
     if (row < a.V0 && y < a.n0) {
       double* drow = a.dst + y * a.n1;
-      if (okA) drow[cA] = yr[v];
-      if (okB) drow[cB] = yi[v];
+      if (okA) put_out(drow + cA, yr[v]);
+      if (okB) put_out(drow + cB, yi[v]);
     }
   }
 }
@@ -757,8 +762,8 @@ __device__ __forceinline__ void store_pair_m(const TileArgs& a, const PairCoord&
       const int row = 4 * q + v;
       if (row >= a.h0 && row < a.h0 + a.V0 && pc.r0 + row - a.h0 < a.n0) {
         double* drow = base + (int64_t)v * a.n1;
-        if (okA) drow[cA] = (double)yr[v] + addA;
-        if (okB) drow[cB] = (double)yi[v] + addB;
+        if (okA) put_out(drow + cA, (double)yr[v] + addA);
+        if (okB) put_out(drow + cB, (double)yi[v] + addB);
       }
     }
   } else {
@@ -769,8 +774,8 @@ __device__ __forceinline__ void store_pair_m(const TileArgs& a, const PairCoord&
         const int n = (225 * j + 16 * row) % 240;
         if (n >= a.h1 && n < a.h1 + a.V1) {
           const int64_t pA = pc.orgA + n - a.h1, pB = pc.orgB + n - a.h1;
-          if (pA < a.n1) a.dst[pA] = (double)yr[v] + addA;
-          if (pc.hasB && pB < a.n1) a.dst[pB] = (double)yi[v] + addB;
+          if (pA < a.n1) put_out(a.dst + pA, (double)yr[v] + addA);
+          if (pc.hasB && pB < a.n1) put_out(a.dst + pB, (double)yi[v] + addB);
         }
       }
     }
@@ -1001,8 +1006,8 @@ __device__ __forceinline__ void store32(const TileArgs& a, const PairCoord& pc, 
         const int64_t y = pc.r0 + row - a.h0;
         if (row >= a.h0 && row < a.h0 + a.V0 && y < a.n0) {
           double* drow = a.dst + y * a.n1;
-          if (okA) drow[cA] = (double)yr.b[2 * R + C][v] * scale + addA;
-          if (okB) drow[cB] = (double)yi.b[2 * R + C][v] * scale + addB;
+          if (okA) put_out(drow + cA, (double)yr.b[2 * R + C][v] * scale + addA);
+          if (okB) put_out(drow + cB, (double)yi.b[2 * R + C][v] * scale + addB);
         }
       }
   }
@@ -1021,8 +1026,8 @@ __device__ __forceinline__ void store16(const TileArgs& a, const PairCoord& pc, 
     const int64_t y = pc.r0 + row;
     if (row < a.V0 && y < a.n0) {
       double* drow = a.dst + y * a.n1;
-      if (okA) drow[cA] = (double)yr[v] * scale + addA;
-      if (okB) drow[cB] = (double)yi[v] * scale + addB;
+      if (okA) put_out(drow + cA, (double)yr[v] * scale + addA);
+      if (okB) put_out(drow + cB, (double)yi[v] * scale + addB);
     }
   }
 }
@@ -1675,8 +1680,8 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
           const int64_t y1 = o1 + n1 - a.h1;
           if (n1 >= a.h1 && n1 < a.h1 + a.V1 && y1 < a.n1) {
             double* row = a.dst + (y0 * a.n1 + y1) * a.n2;
-            if (okA) row[zA] = (double)br[v] + cA * a.dc_gain;
-            if (okB) row[zB] = (double)bi[v] + cB * a.dc_gain;
+            if (okA) put_out(row + zA, (double)br[v] + cA * a.dc_gain);
+            if (okB) put_out(row + zB, (double)bi[v] + cB * a.dc_gain);
           }
         }
       }
