@@ -2028,6 +2028,22 @@ hipError_t spectrum_to_f32(const double* H, float* Hf, int64_t len, hipStream_t 
   return hipGetLastError();
 }
 
+// The 16-point kernels (1D and 2D) stride persistently over their pairs.  They are launched with at most
+// the workgroups that can be resident at once: workgroups beyond that would wait for a later round and
+// start cold, while resident waves take further pairs with the next-pair prefetch (the 1D kernel, 176
+// VGPRs, 2 workgroups per CU: Heat-1D 2^20 x 64 17.3 -> 14.9 us).  Residency is read from the compiled
+// kernel once per kernel.
+template <void (*K)(TileArgs)>
+void launch16(const TileArgs& a, int64_t blocks, int num_cus, hipStream_t s) {
+  static const int resident = [] {  // thread-safe one-time initialisation
+    int nb = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, K, 256, 0);
+    return (e == hipSuccess && nb >= 1) ? nb : 8;
+  }();
+  const int64_t nb = std::min<int64_t>(blocks, (int64_t)num_cus * resident);
+  hipLaunchKernelGGL(K, dim3((unsigned)nb), dim3(256), 0, s, a);
+}
+
 hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus) {
   if (a.pairs <= 0) return hipSuccess;
   if (kind == KIND_3D16) {  // one workgroup per tile pair, two resident per CU (68 KB LDS each for FP64)
@@ -2047,34 +2063,23 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
   // in LDS per workgroup and run 3 waves per SIMD (4 for the FP16 out16 kernel), so they launch exactly
   // the resident workgroups.
   const int64_t per_cu32 = (a.precision == PREC_F16 && a.out16) ? 4 : (a.precision == 0 ? 2 : 3);
-  int64_t cap = (int64_t)num_cus * (kind == KIND_2D32 ? per_cu32 : 8);
-  if (kind == KIND_1DPFA) {
-    // The 1D kernel holds two operand sets (176 VGPRs): launch only the workgroups that can be resident,
-    // so waves take further pairs with the next-pair prefetch instead of later workgroups starting
-    // cold (Heat-1D 2^20 x 64: 17.3 -> 15.0 us).  Residency from the compiled kernel, once.
-    static const int resident = [] {  // thread-safe one-time initialisation
-      int nb = 0;
-      const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tile16_f64<KIND_1DPFA>, 256, 0);
-      return (e == hipSuccess && nb >= 1) ? nb : 8;
-    }();
-    cap = std::min<int64_t>(cap, (int64_t)num_cus * resident);
-  }
+  const int64_t cap = (int64_t)num_cus * (kind == KIND_2D32 ? per_cu32 : 8);
   if (blocks > cap) blocks = cap;
   const dim3 g((unsigned)blocks), b(256);
   if (kind == KIND_2D16) {
     if (a.precision == PREC_BF16)
-      hipLaunchKernelGGL((k_tile16_mixed<KIND_2D16, PREC_BF16, 3>), g, b, 0, s, a);
+      launch16<k_tile16_mixed<KIND_2D16, PREC_BF16, 3>>(a, blocks, num_cus, s);
     else if (a.precision == PREC_F32)
-      hipLaunchKernelGGL((k_tile16_mixed<KIND_2D16, PREC_F32, 1>), g, b, 0, s, a);
+      launch16<k_tile16_mixed<KIND_2D16, PREC_F32, 1>>(a, blocks, num_cus, s);
     else
-      hipLaunchKernelGGL(k_tile16_f64<KIND_2D16>, g, b, 0, s, a);
+      launch16<k_tile16_f64<KIND_2D16>>(a, blocks, num_cus, s);
   } else if (kind == KIND_1DPFA) {
     if (a.precision == PREC_BF16)
-      hipLaunchKernelGGL((k_tile16_mixed<KIND_1DPFA, PREC_BF16, 3>), g, b, 0, s, a);
+      launch16<k_tile16_mixed<KIND_1DPFA, PREC_BF16, 3>>(a, blocks, num_cus, s);
     else if (a.precision == PREC_F32)
-      hipLaunchKernelGGL((k_tile16_mixed<KIND_1DPFA, PREC_F32, 1>), g, b, 0, s, a);
+      launch16<k_tile16_mixed<KIND_1DPFA, PREC_F32, 1>>(a, blocks, num_cus, s);
     else
-      hipLaunchKernelGGL(k_tile16_f64<KIND_1DPFA>, g, b, 0, s, a);
+      launch16<k_tile16_f64<KIND_1DPFA>>(a, blocks, num_cus, s);
   } else if (kind == KIND_2D32) {
     if (a.precision == PREC_BF16 && a.out16)
       hipLaunchKernelGGL((k_tile32_bf16<3, true>), g, b, 0, s, a);
