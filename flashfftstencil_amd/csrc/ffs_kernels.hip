@@ -237,20 +237,35 @@ __device__ __forceinline__ void store_pair(const TileArgs& a, const PairCoord& p
 template <int KIND>
 __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
   // The spectrum (256 complex, 4 KB) is staged in LDS once per workgroup and read per tile pair,
-  // which keeps the kernel within 128 VGPRs (4 waves per SIMD).
+  // which keeps the kernel within 128 VGPRs (4 waves per SIMD).  The 1D kind also stages its second
+  // (15-point) operand set there and reads it just before each of its two stages, and does not prefetch
+  // the next pair: its waves take about one pair each, and under 128 VGPRs four workgroups per CU are
+  // resident instead of two.
+  constexpr bool ONE_D = KIND != KIND_2D16;
   __shared__ double2 sH[256];
+  __shared__ double sO1[ONE_D ? kOpsPerAxis : 1];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) sH[i] = double2{a.H[i], a.H[256 + i]};
+  if (ONE_D)
+    for (int i = threadIdx.x; i < kOpsPerAxis; i += blockDim.x) sO1[i] = a.ops[kOpsPerAxis + i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4, j = lane & 15;
   const int64_t W = (int64_t)gridDim.x * 4;
   const int64_t gw = block_order(a.linear_map, blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
-  // 2D: both axes are 16-point DFTs and share one operand set; 1D: 15-point rows, 16-point columns.
-  OpsAxis f0, f1;
+  // 2D: both axes are 16-point DFTs and share one operand set; 1D: 15-point rows (from LDS at each use),
+  // 16-point columns.
+  OpsAxis f0;
   load_ops(f0, a.ops, 0, lane);
-  if (KIND != KIND_2D16) load_ops(f1, a.ops, 1, lane);
-  const OpsAxis& g1 = (KIND == KIND_2D16) ? f0 : f1;
+  auto stage1 = [&](const double (&ar)[4], const double (&ai)[4], d4& outr, d4& outi, auto inv) {
+    if (ONE_D) {
+      OpsAxis g;
+      load_ops(g, sO1, 0, lane);
+      cstage<decltype(inv)::value>(ar, ai, g, outr, outi);
+    } else {
+      cstage<decltype(inv)::value>(ar, ai, f0, outr, outi);
+    }
+  };
 
   int64_t tp = gw;
   if (tp >= a.pairs) return;
@@ -258,13 +273,13 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
   double xr[4], xi[4];
   gather_pair<KIND>(a, pc, q, j, xr, xi);
   for (; tp < a.pairs; tp += W) {
-    // prefetch the next pair while this one computes
+    // 2D: prefetch the next pair while this one computes
     const int64_t tn = tp + W;
     PairCoord pn = pc;
     double nr[4] = {0.0, 0.0, 0.0, 0.0}, ni[4] = {0.0, 0.0, 0.0, 0.0};
     if (tn < a.pairs) {
       pn = pair_coord<KIND>(a, tn);
-      gather_pair<KIND>(a, pn, q, j, nr, ni);
+      if (!ONE_D) gather_pair<KIND>(a, pn, q, j, nr, ni);
     }
 
     d4 sr, si;
@@ -272,7 +287,7 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
     double cr[4], ci[4];
     to_chunks(sr, cr);
     to_chunks(si, ci);
-    cstage<false>(cr, ci, g1, sr, si);  // contract axis 1 -> Y[k0 = q+4v][k1 = j]
+    stage1(cr, ci, sr, si, std::false_type{});  // contract axis 1 -> Y[k0 = q+4v][k1 = j]
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const double2 h = sH[v * kLanes + lane];
@@ -286,14 +301,18 @@ __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
     cstage<true>(cr, ci, f0, sr, si);  // inverse over k0
     to_chunks(sr, cr);
     to_chunks(si, ci);
-    cstage<true>(cr, ci, g1, sr, si);  // inverse over k1 -> X'[n0 = q+4v][n1 = j]
+    stage1(cr, ci, sr, si, std::true_type{});  // inverse over k1 -> X'[n0 = q+4v][n1 = j]
     store_pair<KIND>(a, pc, q, j, sr, si);
 
     pc = pn;
+    if (ONE_D) {
+      if (tn < a.pairs) gather_pair<KIND>(a, pc, q, j, xr, xi);
+    } else {
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      xr[c] = nr[c];
-      xi[c] = ni[c];
+      for (int c = 0; c < 4; c++) {
+        xr[c] = nr[c];
+        xi[c] = ni[c];
+      }
     }
   }
 }
