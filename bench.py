@@ -164,7 +164,7 @@ def cpu_threads():
 
 
 CPU_SWEEP = ("FP64 direct periodic sweep, register-blocked AVX2 FMAs (oracle/sweep_fast.c, equal to the "
-             "checker sweep oracle/oracle.c to 1e-13)")
+             "oracle's strict sweep oracle/oracle.c to 1e-13)")
 
 
 def oracle_sample(x, radius, w, T, seconds, label):

@@ -1,6 +1,6 @@
 /*
  * oracle.c -- FP64 direct periodic stencil sweep.  TEST INFRASTRUCTURE ONLY: this file is the
- * checker for libffstencil and is never linked into, loaded by or called from the product path.
+ * reference libffstencil is tested against and is never linked into, loaded by or called from the product path.
  * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it.
  *
  * What it restates (KevinWu2017/FlashFFTStencil):

@@ -1,4 +1,4 @@
-"""ctypes access to oracle/liboracle.so (the FP64 direct-sweep checker) and helpers shared by tests.
+"""ctypes access to oracle/liboracle.so (the FP64 direct-sweep oracle) and helpers shared by tests.
 
 Test infrastructure only; see oracle/oracle.c for what it restates from the reference.
 """

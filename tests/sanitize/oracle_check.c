@@ -1,10 +1,10 @@
-/* oracle_check.c -- runs the oracle (oracle/oracle.c, the test-side FP64 checker, and
+/* oracle_check.c -- runs the oracle (oracle/oracle.c, the strict FP64 sweep, and
  * oracle/sweep_fast.c, the timed CPU baseline) under AddressSanitizer and UndefinedBehaviorSanitizer over
  * the shapes the tests use: 1D/2D/3D, ragged and tiny grids (sides smaller than the stencil, so every
  * wrap branch runs; rows shorter and longer than the fast sweep's blocks and strips), both conventions,
- * T = 0, in == out aliasing and rejected arguments.  Each checker run is checked for mass conservation
+ * T = 0, in == out aliasing and rejected arguments.  Each strict-sweep run is checked for mass conservation
  * (sum out = sum in * (sum w)^T, exact up to rounding for periodic grids) and each fast run against the
- * checker's result.  Built and run by tests/test_sanitizers.py. */
+ * strict sweep's result.  Built and run by tests/test_sanitizers.py. */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -40,7 +40,7 @@ static void one(int dims, const int64_t* sizes, int radius, int convention, int6
     failures++;
   }
   for (int64_t i = 0; i < n; i++) sy += dst[i];
-  /* the fast sweep: in place when the checker ran in place, else into a separate buffer */
+  /* the fast sweep: in place when the strict sweep ran in place, else into a separate buffer */
   double* zo = alias ? z : malloc(sizeof(double) * n);
   if (orc_sweep_fast(dims, sizes, radius, w, convention, steps, z, zo, 2) != 0) {
     fprintf(stderr, "fast sweep failed: dims %d radius %d\n", dims, radius);
@@ -52,7 +52,7 @@ static void one(int dims, const int64_t* sizes, int radius, int convention, int6
     rmax = fmax(rmax, fabs(dst[i]));
   }
   if (dmax > 1e-13 * rmax) {
-    fprintf(stderr, "fast vs checker: dims %d radius %d conv %d: %.3g\n", dims, radius, convention, dmax / rmax);
+    fprintf(stderr, "fast vs strict sweep: dims %d radius %d conv %d: %.3g\n", dims, radius, convention, dmax / rmax);
     failures++;
   }
   if (!alias) free(zo);

@@ -191,9 +191,9 @@ def test_t_fold_operator_matches_sweep(oracle, dims, shape, radius):
     (2, [(1, 1), (2, 5), (5, 3), (17, 40), (64, 97), (33, 1100)]),
     (3, [(1, 2, 3), (4, 5, 6), (9, 8, 37), (6, 7, 70)]),
 ])
-def test_fast_sweep_matches_checker(dims, shapes):
+def test_fast_sweep_matches_strict_sweep(dims, shapes):
     """oracle/sweep_fast.c (the timed CPU baseline: register blocks, overlapping tail block, cheap wraps,
-    column strips) against the checker sweep, over rows shorter and longer than its 32-point blocks and
+    column strips) against the strict sweep, over rows shorter and longer than its 32-point blocks and
     512-point strips, grids smaller than the stencil, radii 1..3, zero taps and both conventions."""
     rng = np.random.default_rng(dims)
     for shape in shapes:
