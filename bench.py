@@ -16,7 +16,7 @@ Workloads (seed 0x5EED0000 + index, SplitMix64; weights uniform[0,1) normalised 
 `value` is C3's mixed FP16 form (the metric's headline comparison); every workload's record sits under
 `configs` with its forms' GStencils/s, the dominant kernel's roofline, max-rel / rel-L2 against the FP64
 form (full T) and against the FP64 direct sweep (the oracle, on a bounded sample), and the oracle's CPU
-time on the box's host cores.
+time on the host's cores.
 
 Roofline of the dominant kernel (one fused block = one launch), algorithmic counts only:
   flops  = the real GEMMs the kernel's formulation needs, each counted once (the Gauss 3-GEMM complex
