@@ -244,3 +244,40 @@ def test_graph_cache_cycles_buffer_pairs(ffs, oracle, torch_cuda, precision):
     for i in range(5):
         assert oracle.errors(first[i].cpu().numpy(), oracle.sweep(xs[i], r, w, 48))[1] <= tol
     plan.close()
+
+
+@pytest.mark.parametrize("radius", [2, 3])
+@pytest.mark.parametrize("precision,algorithm", [("fp64", "fft"), ("bf16", "fft"), ("fp64", "auto")])
+@pytest.mark.parametrize("convention", ["ref", "centred"])
+def test_3d_wider_stencils(ffs, oracle, torch_cuda, radius, precision, algorithm, convention):
+    """Custom 3D stencils wider than the named ones (5^3 and 7^3 random weights): the 16^3 tiles then
+    keep V = 16 - 2 r t (down to 4), with tails; ragged grids; against the oracle at the form's bar.
+    AUTO takes the FFT path here (the direct kernels stop at radius 1 in 3D)."""
+    rng = np.random.default_rng(40 + radius)
+    w = rng.random((2 * radius + 1) ** 3)
+    w /= w.sum()
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    for sizes, steps in [((23, 19, 29), 1), ((17, 26, 21), 3), ((30, 14, 33), 5)]:
+        x = oracle.uniform(7 * radius + steps, sizes)
+        try:
+            plan = ffs.Plan(3, sizes, w, radius, steps, precision=precision, algorithm=algorithm,
+                            convention=convention)
+        except ffs.FFSError as e:
+            pytest.fail(f"plan rejected: radius {radius} {precision} {algorithm}: {e}")
+        d_in = torch_cuda.from_numpy(x).cuda()
+        d_out = torch_cuda.empty_like(d_in)
+        plan.run(d_in, d_out)
+        torch_cuda.cuda.synchronize()
+        info = plan.info
+        plan.close()
+        ref = oracle.sweep(x, radius, w, steps, convention=0 if convention == "ref" else 1)
+        rel = oracle.errors(d_out.cpu().numpy(), ref)[1]
+        assert rel <= tol, (radius, precision, algorithm, convention, sizes, steps, info, rel)
+
+
+@pytest.mark.parametrize("radius", [2, 3])
+def test_3d_direct_refuses_wider_stencils(ffs, torch_cuda, radius):
+    """The direct kernels cover radius <= 1 in 3D; wider stencils are a clear FFS_ERR_UNSUPPORTED."""
+    w = np.full((2 * radius + 1) ** 3, 1.0 / (2 * radius + 1) ** 3)
+    with pytest.raises(ffs.FFSError, match="not offered in 3D"):
+        ffs.Plan(3, (20, 20, 20), w, radius, 2, precision="fp64", algorithm="direct")
