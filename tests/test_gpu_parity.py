@@ -281,3 +281,34 @@ def test_3d_direct_refuses_wider_stencils(ffs, torch_cuda, radius):
     w = np.full((2 * radius + 1) ** 3, 1.0 / (2 * radius + 1) ** 3)
     with pytest.raises(ffs.FFSError, match="not offered in 3D"):
         ffs.Plan(3, (20, 20, 20), w, radius, 2, precision="fp64", algorithm="direct")
+
+
+@pytest.mark.parametrize("dims,radius", [(2, 5), (2, 7), (1, 5), (1, 12)])
+@pytest.mark.parametrize("weights", ["signed", "growing"])
+def test_wide_stencils_unusual_weights(ffs, oracle, torch_cuda, dims, radius, weights):
+    """Wide custom stencils in 1D and 2D, every FFT form the dimension offers, with weights that are not
+    a positive normalised set: signed taps summing to 1 (an oscillating kernel) or positive taps summing
+    to 1.3 (a growing field, so the DC gain (sum w)^t and the mixed forms' tile scales are exercised)."""
+    rng = np.random.default_rng(10 * dims + radius)
+    n = (2 * radius + 1) ** dims
+    if weights == "signed":
+        w = rng.random(n) - 0.45
+        w /= w.sum()
+    else:
+        w = rng.random(n)
+        w *= 1.3 / w.sum()
+    sizes = (3001,) if dims == 1 else (71, 90)
+    steps = 6
+    x = oracle.uniform(31 * radius, sizes)
+    ref = oracle.sweep(x, radius, w, steps)
+    forms = ["fp64", "bf16", "f32"] + (["f16"] if dims == 2 else [])
+    for form in forms:
+        plan = ffs.Plan(dims, sizes, w, radius, steps, precision=form)
+        d_in = torch_cuda.from_numpy(x).cuda()
+        d_out = torch_cuda.empty_like(d_in)
+        plan.run(d_in, d_out)
+        torch_cuda.cuda.synchronize()
+        info = plan.info
+        plan.close()
+        rel = oracle.errors(d_out.cpu().numpy(), ref)[1]
+        assert rel <= (1e-12 if form == "fp64" else 1e-6), (dims, radius, weights, form, info, rel)
