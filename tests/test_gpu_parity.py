@@ -312,3 +312,25 @@ def test_wide_stencils_unusual_weights(ffs, oracle, torch_cuda, dims, radius, we
         plan.close()
         rel = oracle.errors(d_out.cpu().numpy(), ref)[1]
         assert rel <= (1e-12 if form == "fp64" else 1e-6), (dims, radius, weights, form, info, rel)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16"])
+def test_3d_mappings_bitwise_equal(ffs, oracle, torch_cuda, precision):
+    """The 3D kernel's wave-to-tile orders (XCD bands: the mixed form's default; hardware order: the FP64
+    form's) only reorder independent tile pairs: bitwise equal results, both against the oracle."""
+    _, r, w, _ = ffs.named_stencil("Box3D27P", seed=5)
+    x = oracle.uniform(77, (40, 24, 30))
+    outs = []
+    for mapping in ("xcd", "linear", "auto"):
+        plan = ffs.Plan(3, x.shape, w, r, 5, precision=precision, mapping=mapping)
+        info = plan.info
+        d_in = torch_cuda.from_numpy(x).cuda()
+        d_out = torch_cuda.empty_like(d_in)
+        plan.run(d_in, d_out)
+        torch_cuda.cuda.synchronize()
+        plan.close()
+        outs.append(d_out.cpu().numpy())
+        if mapping == "auto":
+            assert info["mapping"] == (2 if precision == "fp64" else 1), info
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    assert oracle.errors(outs[0], oracle.sweep(x, r, w, 5))[1] <= (1e-12 if precision == "fp64" else 1e-6)
