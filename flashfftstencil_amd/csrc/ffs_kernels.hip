@@ -237,9 +237,9 @@ __device__ __forceinline__ void store_pair(const TileArgs& a, const PairCoord& p
 template <int KIND>
 __global__ __launch_bounds__(256) void k_tile16_f64(TileArgs a) {
   // The spectrum (256 complex, 4 KB) is staged in LDS once per workgroup and read per tile pair,
-  // which keeps the kernel within 128 VGPRs (4 waves per SIMD).  The 1D kind also stages its second
+  // which keeps the 2D kind within 128 VGPRs (4 waves per SIMD).  The 1D kind also stages its second
   // (15-point) operand set there and reads it just before each of its two stages, and does not prefetch
-  // the next pair: its waves take about one pair each, and under 128 VGPRs four workgroups per CU are
+  // the next pair: its waves take about one pair each, and at 158 VGPRs three workgroups per CU are
   // resident instead of two.
   constexpr bool ONE_D = KIND != KIND_2D16;
   __shared__ double2 sH[256];
@@ -1328,12 +1328,12 @@ __global__ __launch_bounds__(256, 3) void k_tile32_bf16(TileArgs a) {
 }
 
 // Multi-grid launch: one fused block of every grid still running, as one persistent launch over the
-// concatenated work list.  Each item carries its own spectrum, read per tile pair from its FP64 table
-// (L2-resident); with only the DFT operands in LDS this kernel stays near 160 VGPRs and runs 3 waves
-// per SIMD (measured faster for batches than 2 waves).  The single-grid kernel above stages its one
-// spectrum in LDS instead (measured faster for one large grid).  items[g] describes grid g's block (source, destination, depth, spectrum);
-// prefix[g] is the first work index of grid g (prefix[n] = total).  All items cost the same (one 32x32
-// tile pair), so waves take them in a fixed strided order: balanced and bitwise reproducible.
+// concatenated work list.  Each item carries its own spectrum (an FP32 copy for the mixed forms, FP64 for
+// the FP64 form); the DFT operands are staged in LDS and the kernels run 3 waves per SIMD (2 for FP64).
+// items[g] describes grid g's block (source, destination, depth, spectrum); prefix[g] is the first work
+// index of grid g (prefix[n] = total).  All items cost the same (one 32x32 tile pair), so waves take them
+// in a fixed order (strided, or in chunks for the FP16 kernel): balanced and bitwise reproducible.
+//
 // Work-item walk of the FP16 batch kernel.  Each wave takes kBatchChunk consecutive tile pairs, then
 // strides by the wave count times that: the chip-wide front of the launch is still one contiguous band of
 // pairs (neighbouring windows share L2), and a wave usually stays in one grid for a whole chunk.  So it
