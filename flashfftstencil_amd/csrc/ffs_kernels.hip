@@ -2049,9 +2049,9 @@ hipError_t spectrum_to_f32(const double* H, float* Hf, int64_t len, hipStream_t 
 
 // The 16-point kernels (1D and 2D) stride persistently over their pairs.  They are launched with at most
 // the workgroups that can be resident at once: workgroups beyond that would wait for a later round and
-// start cold, while resident waves take further pairs with the next-pair prefetch (the 1D kernel, 176
-// VGPRs, 2 workgroups per CU: Heat-1D 2^20 x 64 17.3 -> 14.9 us).  Residency is read from the compiled
-// kernel once per kernel.
+// start cold, while resident waves take further pairs (measured on the 1D kernel when it held 2
+// workgroups per CU: Heat-1D 2^20 x 64, 17.3 -> 14.9 us).  Residency is read from the compiled kernel,
+// once per kernel.
 template <void (*K)(TileArgs)>
 void launch16(const TileArgs& a, int64_t blocks, int num_cus, hipStream_t s) {
   static const int resident = [] {  // thread-safe one-time initialisation
