@@ -405,8 +405,8 @@ bool out16_block(const ffs_plan* pl) {
 bool linear_map(const ffs_plan* pl) {
   const ffs_problem& p = pl->prob;
   if (p.mapping != FFS_MAP_AUTO) return p.mapping == FFS_MAP_LINEAR;
-  // XCD bands where they measured faster: the 32-point FP64 kernel (4%) and the 3D mixed kernel (0.8%,
-  // since its spectrum moved to LDS); the hardware order elsewhere (2D mixed: XCD bands 4% slower).
+  // XCD bands where they measured faster: the 32-point FP64 kernel (4%) and the 3D mixed kernel (0.8%);
+  // the hardware order elsewhere (2D mixed: XCD bands 4% slower; 3D FP64: 0.5% slower).
   const bool xcd = (pl->kind == ffsk::KIND_2D32 && p.precision == FFS_PREC_FP64) ||
                    (pl->kind == ffsk::KIND_3D16 && p.precision != FFS_PREC_FP64);
   return !xcd;
