@@ -11,7 +11,9 @@ would take minutes, through checks that do not depend on the size:
   * mass: sum(out) = sum(in) (sum w)^T for any weights.
 Bars: FP64 1e-12, mixed forms 1e-6 (max error over the checked points / max |reference| there).  The
 workloads: Box2D25P and Box2D9P 8192^2 (SplitMix64 uniform grids and weights), Box3D27P 512^3 (the
-seeded sinusoid field) and Heat-1D 2^20 (checked against a full oracle sweep, which is cheap in 1D).
+seeded sinusoid field), Heat-1D 2^20 (checked against a full oracle sweep, which is cheap in 1D) and
+the 96-grid batch (FP16 against FP64 on every grid, FP64 bitwise against single-grid plans, the
+smallest grids against the oracle).
 """
 import numpy as np
 import pytest
@@ -159,3 +161,45 @@ def test_heat1d_2p20_full_oracle(ffs, oracle, torch_cuda, form, algorithm):
     torch_cuda.cuda.synchronize()
     err = oracle.errors(d_out.cpu().numpy(), oracle.sweep(x, r, w, steps))[1]
     assert err <= BARS[form], (form, algorithm, err)
+
+
+def test_batch96_full(ffs, oracle, torch_cuda):
+    """The 96-grid batch workload at full size (sides 512..4096, four stencils, T 16..256, one persistent
+    launch per round): the FP16 batch against the FP64 batch on every grid at the mixed bar, the FP64
+    batch bitwise equal to single-grid FP64 plans on one grid of each stencil, and the two smallest grids
+    of the FP64 batch against the oracle."""
+    import bench
+    torch = torch_cuda
+    specs = bench.batch96_specs(ffs, 0x5EED0004)
+    d_in = [torch.from_numpy(ffs.fill_uniform(s["seed"], s["sizes"][0] * s["sizes"][1]).reshape(s["sizes"])).cuda()
+            for s in specs]
+    outs, infos = {}, {}
+    for form in ("fp64", "f16"):
+        b = ffs.Batch(specs, precision=form)
+        d_out = [torch.empty_like(d) for d in d_in]
+        b.run(d_in, d_out)
+        torch.cuda.synchronize()
+        infos[form] = [b.info(i) for i in range(len(specs))]
+        outs[form] = d_out
+        del b
+    for i, s in enumerate(specs):
+        ref, got = outs["fp64"][i], outs["f16"][i]
+        err = float((got - ref).abs().max() / ref.abs().max())
+        assert err <= BARS["f16"], (i, s["shape"], s["sizes"], s["steps"], infos["f16"][i], err)
+    seen = set()
+    for i, s in enumerate(specs):
+        if s["shape"] in seen:
+            continue
+        seen.add(s["shape"])
+        inf = infos["fp64"][i]
+        with ffs.Plan(2, s["sizes"], s["weights"], s["radius"], s["steps"], precision="fp64", tile=32,
+                      fuse=inf["fuse"]) as plan:
+            d_out = torch.empty_like(d_in[i])
+            plan.run(d_in[i], d_out)
+            torch.cuda.synchronize()
+        assert torch.equal(d_out, outs["fp64"][i]), (i, s["shape"], s["sizes"], inf)
+    smallest = sorted(range(len(specs)), key=lambda i: specs[i]["sizes"][0] * specs[i]["sizes"][1] * specs[i]["steps"])[:2]
+    for i in smallest:
+        s = specs[i]
+        ref = oracle.sweep(d_in[i].cpu().numpy(), s["radius"], s["weights"], s["steps"])
+        assert oracle.errors(outs["fp64"][i].cpu().numpy(), ref)[1] <= BARS["fp64"], (i, s["shape"], s["sizes"])
