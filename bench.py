@@ -8,7 +8,7 @@ Workloads (seed 0x5EED0000 + index, SplitMix64; weights uniform[0,1) normalised 
   C1 Heat-1D 1,048,576 x T=64           FP64 FFT form (+ direct FP64)
   C2 Box2D9P 8192^2 x T=128             FP64 FFT form (+ mixed forms, direct)
   C3 Box2D25P 8192^2 x T=256            mixed FP16 form vs FP64 (+ BF16, direct)  <- `value`
-  C4 Box3D27P 512^3 x T=32              sinusoid field; mixed BF16 form vs FP64 (+ direct)
+  C4 Box3D27P 512^3 x T=32              sinusoid field; mixed BF16 form vs FP64 (+ compensated FP32, direct)
   C5 96 2D grids, 512..4096 sides, 4 stencils, T 16..256: one persistent launch per round, fused depths
      and mapping from the batch autotuner (ffs_batch_autotune, timed outside the timed region; the
      planner's depths are timed beside it)
@@ -21,7 +21,8 @@ time on the host's cores.
 Roofline of the dominant kernel (one fused block = one launch), algorithmic counts only:
   flops  = the real GEMMs the kernel's formulation needs, each counted once (the Gauss 3-GEMM complex
            product, the inverse stages cut to the kept output block, never the split products of the
-           mixed forms: executed MFMA flops / products per real GEMM, 1 FP64/FP32, 3 FP16, 6 BF16),
+           mixed forms: executed MFMA flops / products per real GEMM, 1 FP64/FP32 (2 for the
+           compensated FP32 form in 3D), 3 FP16, 6 BF16),
            against the matrix peak of the form's arithmetic (FP64 78.6 TF, FP32 157.3 TF, FP16/BF16
            2.5 PF); direct kernel: 2 |S| t per point against the FP64 vector peak;
   bytes  = 16 x points (FP64 state read once and written once per block) against 8 TB/s;
@@ -55,11 +56,12 @@ WORKLOADS = {
     "box2d25p": dict(cfg="C3", shape="Box2D25P", sizes=(8192, 8192), T=256, seed=0x5EED0002,
                      forms=("f16", "bf16", "fp64", "direct")),
     "box3d27p": dict(cfg="C4", shape="Box3D27P", sizes=(512, 512, 512), T=32, seed=0x5EED0003,
-                     forms=("bf16", "fp64", "direct"), init="sinusoids"),
+                     forms=("bf16", "fp64", "f32", "direct"), init="sinusoids"),
     "batch96": dict(cfg="C5", seed=0x5EED0004, forms=("f16", "bf16", "fp64")),
 }
 ORDER = ("heat1d", "box2d9p", "box2d25p", "box3d27p", "batch96")
-DTYPES = {"fp64": "f64", "bf16": "bf16x3-split/f32-acc (f64 state)", "f32": "f32 (f64 state)",
+DTYPES = {"fp64": "f64", "bf16": "bf16x3-split/f32-acc (f64 state)",
+          "f32": "f32 (f64 state; 3D: compensated matrix, 2 products)",
           "f16": "f16x2-split/f32-acc (f64 state)", "direct": "f64 (direct sum)"}
 
 
@@ -102,6 +104,11 @@ def tile_flops(info):
 PRODUCTS = {"fp64": 1, "f32": 1, "f16": 3, "bf16": 6}  # MFMA products per real GEMM (split terms)
 
 
+def products(form, info):
+    """MFMA products per real GEMM; the 3D FP32 form adds the compensating low-order products."""
+    return 2 if form == "f32" and info["dims"] == 3 else PRODUCTS[form]
+
+
 def roofline(info, kernel_s, form, radius, taps):
     """Roofline object of one fused block (one launch) timed at kernel_s seconds."""
     pts = float(np.prod(info["sizes"][:info["dims"]]))
@@ -111,7 +118,7 @@ def roofline(info, kernel_s, form, radius, taps):
         flops = 2.0 * taps * pts * info["fuse"]
         peak, unit_peak = VECTOR_PEAK_FP64, "fp64 vector"
     else:
-        flops = info["flops_per_block"] / PRODUCTS[form]
+        flops = info["flops_per_block"] / products(form, info)
         dense = tile_flops(info) * info["tiles"]
         peak, unit_peak = MATRIX_PEAK[form], f"{form} matrix"
     f_frac = flops / kernel_s / peak
@@ -402,7 +409,7 @@ def bench_batch(args, torch, ffs, forms=None):
         for i in range(len(specs)):
             inf = batch.info(i)
             rounds = max(rounds, inf["blocks"])
-            flops += inf["flops_per_block"] / PRODUCTS[form] * inf["blocks"]
+            flops += inf["flops_per_block"] / products(form, inf) * inf["blocks"]
             bytes_alg += 16.0 * inf["sizes"][0] * inf["sizes"][1] * inf["blocks"]
         s = ms * 1e-3
         f_frac, b_frac = flops / s / MATRIX_PEAK[form], bytes_alg / s / HBM_PEAK

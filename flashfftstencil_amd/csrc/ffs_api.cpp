@@ -250,13 +250,6 @@ ffs_status choose_config(ffs_plan* pl) {
   }
   if (p.precision == FFS_PREC_MIXED_F16 && p.dims != 2)
     return fail(FFS_ERR_UNSUPPORTED, "the FP16 mixed form is offered for 2D grids (32-point tiles) only");
-  // Six FP32-operand stages per block round the accumulator every 4 products; on smooth 3D fields
-  // (Box3D27P 512^3 x 32, sinusoid field) that measured 1.3e-6 > 1e-6.  A compensated form needs two or
-  // more FP32 products per real GEMM, which on MI355X (FP32 matrix rate = 2x FP64) costs as much as the
-  // FP64 form's single product: 3D runs FP64 or the BF16 split.
-  if (p.precision == FFS_PREC_MIXED_F32 && p.dims == 3)
-    return fail(FFS_ERR_UNSUPPORTED, "the FP32 mixed form is offered for 1D and 2D grids; in 3D it misses 1e-6 "
-                                     "(use FFS_PREC_MIXED_BF16 or FFS_PREC_FP64)");
   if (p.dims == 1) {
     if (p.tile != 0 && p.tile != 240)
       return fail(FFS_ERR_UNSUPPORTED, "1D tile must be 240 (15x16 prime-factor window), got %d", p.tile);
@@ -322,7 +315,9 @@ ffs_status choose_config(ffs_plan* pl) {
         pairs *= d == p.dims - 1 ? std::ceil(tiles / 2.0) : tiles;
       }
       const bool out16 = kind == ffsk::KIND_2D32 && V <= 16 && !(p.flags & FFS_FLAG_FULL_STAGES);
-      const double c = compute * (p.dims == 3 ? 16.0 : 1.0) * (out16 ? cut : 1.0);
+      // 3D: 16 slices per pair; its FP32 form is the compensated one, two products per real GEMM
+      const double c = compute * (p.dims == 3 ? (p.precision == FFS_PREC_MIXED_F32 ? 32.0 : 16.0) : 1.0) *
+                       (out16 ? cut : 1.0);
       const double cycles = (double)nb * (launch + pairs * std::max(c, hbm_per_point * Vd) / 1024.0);
       const double score = points * (double)T / cycles;
       if (score > best * (1 + 1e-9)) {
@@ -907,12 +902,13 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   // GEMMs per complex product x MFMA flops per real GEMM.
   //   FP64: Gauss form, 3 GEMMs of 4 x v_mfma_f64_16x16x4f64 (2*16^3 flops each GEMM)
   //   BF16: 4 GEMMs of NTERMS x v_mfma_f32_16x16x32_bf16 (2*16*16*32 flops each instruction)
-  //   FP32: 4 GEMMs of 4 x v_mfma_f32_16x16x4f32 (2*16^3 flops each GEMM)
+  //   FP32: 4 GEMMs of 4 x v_mfma_f32_16x16x4f32 (2*16^3 flops each GEMM); twice that in 3D, whose
+  //         FP32 form adds the products with the matrix's low-order terms
   const double gemm = 2.0 * 16 * 16 * 16;
   const int dd = p.dims == 1 ? 2 : p.dims;
   double per_cplx = 3.0 * gemm;
   if (p.precision == FFS_PREC_MIXED_BF16) per_cplx = 4.0 * 3.0 * 2.0 * 16 * 16 * 32;
-  if (p.precision == FFS_PREC_MIXED_F32) per_cplx = 4.0 * gemm;
+  if (p.precision == FFS_PREC_MIXED_F32) per_cplx = (p.dims == 3 ? 8.0 : 4.0) * gemm;
   double per_pair = 2.0 * dd * per_cplx * std::pow(16.0, dd - 2);
   // 32-point family: output blocks over the 4 stages (4 + 4 + 4 + 4, or 4 + 4 + 2 + 1 for an out16 main
   // block) x 4 real GEMMs x (6, 4 or 3) K=32 MFMAs

@@ -25,12 +25,15 @@ constexpr int kOpsLen = 3 * kOpsPerAxis;     // 2304 doubles
 //     element e of lane (j + 16q): row 4q + (e & 3), column j, term (e < 4 ? first : second) of
 //     g = 0: (Fh | Fh), g = 1: (Fm | Fm), g = 2: (Fl | Fh), g = 3: (Fm | Fl)
 //     where F = Fh + Fm + Fl is the 3-term BF16 split.
-//   FP32 B operands [axis][arr][chunk c][lane] = F[4(lane/16) + c][lane % 16]
+//   FP32 B operands [axis][arr][chunk c][lane] = F[4(lane/16) + c][lane % 16], rounded to FP32,
+//   then the same layout again with the low-order terms fl32(F - fl32(F)) (the 3D FP32 form's
+//   compensating products)
 constexpr int kPairings = 4;
 constexpr int kBf16WordsPerAxis = 3 * kPairings * 4 * kLanes;  // 3072
 constexpr int kF32PerAxis = 3 * 4 * kLanes;            // 768
-constexpr int kOpsMixedLen = 3 * kBf16WordsPerAxis + 3 * kF32PerAxis;
+constexpr int kOpsMixedLen = 3 * kBf16WordsPerAxis + 2 * 3 * kF32PerAxis;
 constexpr int kOpsF32Offset = 3 * kBf16WordsPerAxis;
+constexpr int kOpsF32LoOffset = kOpsF32Offset + 3 * kF32PerAxis;
 
 inline int64_t spectrum_len(int kind) {  // complex values per spectrum (one depth)
   return kind == KIND_3D16 ? 16 * 16 * 16 : (kind == KIND_2D32 ? 32 * 32 : 16 * 16);

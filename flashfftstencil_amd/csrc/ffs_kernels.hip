@@ -623,6 +623,9 @@ struct OpsBF16 {
 struct OpsF32 {
   float r[4], i[4], ni[4];  // [chunk]
 };
+struct OpsF32C {  // FP32 operands and their low-order terms, F = hi + lo to ~2^-48
+  OpsF32 hi, lo;
+};
 
 __device__ __forceinline__ bf8 as_bf8(u4 w) { return __builtin_bit_cast(bf8, w); }
 
@@ -644,14 +647,22 @@ __device__ __forceinline__ void load_ops_mixed(OpsBF16<NTERMS>& o, const uint32_
     }
 }
 
-__device__ __forceinline__ void load_ops_mixed(OpsF32& o, const uint32_t* __restrict__ t, int axis, int lane) {
-  const float* base = reinterpret_cast<const float*>(t + kOpsF32Offset) + axis * kF32PerAxis;
+__device__ __forceinline__ void load_ops_f32(OpsF32& o, const uint32_t* __restrict__ t, int offset, int axis,
+                                             int lane) {
+  const float* base = reinterpret_cast<const float*>(t + offset) + axis * kF32PerAxis;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     o.r[c] = base[(0 * 4 + c) * kLanes + lane];
     o.i[c] = base[(1 * 4 + c) * kLanes + lane];
     o.ni[c] = base[(2 * 4 + c) * kLanes + lane];
   }
+}
+__device__ __forceinline__ void load_ops_mixed(OpsF32& o, const uint32_t* __restrict__ t, int axis, int lane) {
+  load_ops_f32(o, t, kOpsF32Offset, axis, lane);
+}
+__device__ __forceinline__ void load_ops_mixed(OpsF32C& o, const uint32_t* __restrict__ t, int axis, int lane) {
+  load_ops_f32(o.hi, t, kOpsF32Offset, axis, lane);
+  load_ops_f32(o.lo, t, kOpsF32LoOffset, axis, lane);
 }
 
 typedef float f2 __attribute__((ext_vector_type(2)));
@@ -709,9 +720,8 @@ __device__ __forceinline__ void mstage(const f4& ar, const f4& ai, const OpsBF16
   outi = im;
 }
 
-template <bool INV, int NTERMS>
-__device__ __forceinline__ void mstage(const f4& ar, const f4& ai, const OpsF32& f, f4& outr, f4& outi) {
-  f4 re = {0.f, 0.f, 0.f, 0.f}, im = re;
+template <bool INV>
+__device__ __forceinline__ void mstage_f32(const f4& ar, const f4& ai, const OpsF32& f, f4& re, f4& im) {
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     re = FFS_MFMA_F32(ar[c], f.r[c], re);
@@ -719,6 +729,23 @@ __device__ __forceinline__ void mstage(const f4& ar, const f4& ai, const OpsF32&
     re = FFS_MFMA_F32(ai[c], INV ? f.i[c] : f.ni[c], re);
     im = FFS_MFMA_F32(ai[c], f.r[c], im);
   }
+}
+template <bool INV, int NTERMS>
+__device__ __forceinline__ void mstage(const f4& ar, const f4& ai, const OpsF32& f, f4& outr, f4& outi) {
+  f4 re = {0.f, 0.f, 0.f, 0.f}, im = re;
+  mstage_f32<INV>(ar, ai, f, re, im);
+  outr = re;
+  outi = im;
+}
+// Compensated FP32 stage: the products with the matrix's low-order terms first (about 2^-24 of the
+// result, so their roundings are negligible), then those with its FP32-rounded entries, into the same
+// FP32 accumulators.  The matrix then enters to ~2^-48 instead of FP32's 2^-25: its rounding is the same
+// in every window and every block, so uncompensated it adds up coherently over a run.
+template <bool INV, int NTERMS>
+__device__ __forceinline__ void mstage(const f4& ar, const f4& ai, const OpsF32C& f, f4& outr, f4& outi) {
+  f4 re = {0.f, 0.f, 0.f, 0.f}, im = re;
+  mstage_f32<INV>(ar, ai, f.lo, re, im);
+  mstage_f32<INV>(ar, ai, f.hi, re, im);
   outr = re;
   outi = im;
 }
@@ -1497,6 +1524,21 @@ struct Prec3<PREC_BF16, NTERMS> {
   static __device__ __forceinline__ void load_ops(Ops& f, const TileArgs& a, int lane) { load_ops_mixed(f, a.opsm, 0, lane); }
 };
 
+template <int NTERMS>
+struct Prec3<PREC_F32, NTERMS> {  // compensated FP32 (two products per real GEMM)
+  typedef float V;
+  typedef f4 V4;
+  typedef OpsF32C Ops;
+  struct C2 {
+    float re, im;
+  };
+  template <bool INV>
+  static __device__ __forceinline__ void stage(const V4& ar, const V4& ai, const Ops& f, V4& outr, V4& outi) {
+    mstage<INV, NTERMS>(ar, ai, f, outr, outi);
+  }
+  static __device__ __forceinline__ void load_ops(Ops& f, const TileArgs& a, int lane) { load_ops_mixed(f, a.opsm, 0, lane); }
+};
+
 // Geometry of 3D tile pair tp: tile origins (o*), window origins (w*), whether tile B exists and
 // whether the window needs no periodic wrap.
 struct Win3 {
@@ -1546,21 +1588,26 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
   typedef typename P::V4 V4;
   typedef typename P::C2 C2;
   __shared__ C2 sX[16 * 16 * 17];
-  // Mixed form: planes k2 = 0..8 of the FP32 spectrum in LDS (18 KB; with the window 52 KB, 3 workgroups
+  // BF16 form: planes k2 = 0..8 of the FP32 spectrum in LDS (18 KB; with the window 52 KB, 3 workgroups
   // per CU).  Real taps make the spectrum Hermitian, H(-k mod 16) = conj H(k), so planes 9..15 are read
   // as conjugates of planes 7..1: the same FP32 values as the global table, without its L2 latency in
-  // phase 2.  First read in phase 2, after the barriers of the window fill.
-  __shared__ f2 sH[PREC != 0 ? 9 * 4 * kLanes : 1];
+  // phase 2.  First read in phase 2, after the barriers of the window fill.  The compensated FP32 form
+  // multiplies by the FP64 spectrum instead (in FP64, rounded once to FP32), from global memory.
+  constexpr bool H64 = PREC == PREC_F32;
+  constexpr bool HLDS = PREC != 0 && !H64;
+  __shared__ f2 sH[HLDS ? 9 * 4 * kLanes : 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int q = lane >> 4, j = lane & 15;
   typename P::Ops f;
   P::load_ops(f, a, lane);
   const bool mixed = PREC != 0;
-  if (mixed)
+  if (HLDS)
     for (int i = threadIdx.x; i < 9 * 4 * kLanes; i += blockDim.x) sH[i] = reinterpret_cast<const f2*>(a.Hf)[i];
-  // Phase loops: fully unrolled in the mixed form (later slices' loads issue early, 1.5% faster), two
-  // slices at a time in the FP64 form (full unrolling costs it 3% at its 2 waves per SIMD).
+  // Phase loops: fully unrolled in the mixed forms (later slices' loads issue early, 1.5% faster), two
+  // slices at a time in the FP64 form (full unrolling costs it 3% at its 2 waves per SIMD) and in phase 2
+  // of the FP32 form (fully unrolled, its FP64 spectrum loads take it to 178 VGPRs, 2 waves per SIMD).
   constexpr int kSlices = PREC == 0 ? 2 : 4;
+  constexpr int kSlices2 = H64 ? 2 : kSlices;
 
   // Window of the pair: every thread (y = tid / 16, z = tid % 16) loads column z of row y in all 16
   // slices x, 32 loads in flight.  With PREFETCH (FP64) the next pair's window is loaded right after
@@ -1633,7 +1680,7 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
       }
     }
     __syncthreads();
-#pragma unroll kSlices
+#pragma unroll kSlices2
     for (int s = 0; s < 4; s++) {
       const int k2 = 4 * w + s;
       V4 br, bi;
@@ -1642,6 +1689,13 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
       const double* H = a.H + (int64_t)k2 * 4 * kLanes + lane;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
+        if (H64) {  // FP64 product with the FP64 spectrum, one rounding to FP32
+          double yr = br[v], yi = bi[v];
+          cmul(yr, yi, H[v * kLanes], H[4096 + v * kLanes]);
+          br[v] = (V)yr;
+          bi[v] = (V)yi;
+          continue;
+        }
         V hr, hi;
         if (mixed) {  // the FP32 copy, from LDS; k2 is wave-uniform, so is the branch
           if (k2 <= 8) {
@@ -1781,13 +1835,16 @@ __global__ void k_gen_operands_mixed(uint32_t* t, int32_t L0, int32_t L1, int32_
       t[axis * kBf16WordsPerAxis + ((arr * kPairings + g) * 4 + w) * kLanes + lane] = word;
     }
   }
-  // FP32 chunks: chunk c of lane (j + 16q) = F[4q + c][j]
+  // FP32 chunks: chunk c of lane (j + 16q) = F[4q + c][j], and its low-order term F - fl32(F)
   float* tf = reinterpret_cast<float*>(t + kOpsF32Offset) + axis * kF32PerAxis;
+  float* tl = reinterpret_cast<float*>(t + kOpsF32LoOffset) + axis * kF32PerAxis;
   for (int c = 0; c < 4; c++) {
     double fr, fi;
     dft_entry(4 * q + c, j, L, fr, fi);
     const double v = arr == 0 ? fr : (arr == 1 ? fi : -fi);
-    tf[(arr * 4 + c) * kLanes + lane] = (float)v;
+    const float hi = (float)v;
+    tf[(arr * 4 + c) * kLanes + lane] = hi;
+    tl[(arr * 4 + c) * kLanes + lane] = (float)(v - (double)hi);
   }
 }
 
@@ -2070,9 +2127,11 @@ hipError_t launch_block(const TileArgs& a, int kind, hipStream_t s, int num_cus)
     const dim3 g3((unsigned)blocks3), b3(256);
     if (a.precision == PREC_BF16)
       hipLaunchKernelGGL((k_tile3d<PREC_BF16, 3>), g3, b3, 0, s, a);
+    else if (a.precision == PREC_F32)
+      hipLaunchKernelGGL((k_tile3d<PREC_F32, 1>), g3, b3, 0, s, a);
     else if (a.precision == 0)
       hipLaunchKernelGGL((k_tile3d<0, 1>), g3, b3, 0, s, a);
-    else  // the FP32 form is not offered in 3D (the planner rejects it)
+    else
       return hipErrorInvalidValue;
     return hipGetLastError();
   }

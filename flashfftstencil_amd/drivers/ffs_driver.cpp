@@ -50,7 +50,7 @@ static void print_help() {
       "  --custom FILE|-      read (2r+1)^%d stencil weights (row-major, ref convention) from FILE or stdin;\n"
       "                       r is the shape's radius\n"
       "  --precision P        fp64 (default) | bf16 (mixed: BF16 split operands) | f16 (mixed: FP16 split\n"
-      "                       operands, 2D) | f32 (mixed: FP32 operands, 1D and 2D)\n"
+      "                       operands, 2D) | f32 (mixed: FP32 operands; compensated DFT matrix in 3D)\n"
       "  --convention C       ref (default, out[p] = sum_a w[a] in[p-a]) | centred (out[p] = sum_o w[o+r] in[p+o])\n"
       "  --algorithm A        fft (default: tiled FFT stencil) | direct (FP64 stencil sum, temporal blocking) |\n"
       "                       auto (direct where offered)\n"

@@ -54,8 +54,9 @@ typedef enum ffs_status {
 typedef enum ffs_precision {
   FFS_PREC_FP64 = 0,       /* FP64 MFMA (v_mfma_f64_16x16x4f64) contractions: the drop-in form */
   FFS_PREC_MIXED_BF16 = 1,  /* 3-term BF16 split operands, FP32 accumulation (v_mfma_f32_16x16x32_bf16) */
-  FFS_PREC_MIXED_F32 = 2,   /* FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4f32); 1D and 2D only
-                               (FFS_ERR_UNSUPPORTED in 3D, where six stages per block miss 1e-6) */
+  FFS_PREC_MIXED_F32 = 2,   /* FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4f32); in 3D the DFT
+                               matrix is compensated (FP32 entries plus their low-order FP32 terms, two
+                               products per real GEMM) */
   FFS_PREC_MIXED_BF16X2 = 3, /* withdrawn (FFS_ERR_UNSUPPORTED): a 2-term BF16 data split (16 bits) misses 1e-6
                                 before the field has smoothed, and was not faster than the 3-term form */
   FFS_PREC_MIXED_F16 = 4     /* 2-term FP16 split operands (22 bits) with a per-tile power-of-2 scale, FP32

@@ -119,12 +119,12 @@ static void validation() {
          q.weights = w3.data();
          q.precision = FFS_PREC_MIXED_F16;
        }},
-      {"f32 in 3D", [](ffs_problem& q) {
-         static std::vector<double> w3(27, 1.0 / 27.0);
-         q.dims = 3;
-         q.sizes[2] = 16;
-         q.weights = w3.data();
-         q.precision = FFS_PREC_MIXED_F32;
+      {"f16 in 1D", [](ffs_problem& q) {
+         static std::vector<double> w1(3, 1.0 / 3.0);
+         q.dims = 1;
+         q.radius = 1;
+         q.weights = w1.data();
+         q.precision = FFS_PREC_MIXED_F16;
        }},
       {"fuse too deep", [](ffs_problem& q) { q.fuse = 40; q.tile = 16; }},
   };

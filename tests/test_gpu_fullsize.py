@@ -11,7 +11,7 @@ would take minutes, through checks that do not depend on the size:
   * mass: sum(out) = sum(in) (sum w)^T for any weights.
 Bars: FP64 1e-12, mixed forms 1e-6 (max error over the checked points / max |reference| there).  The
 workloads: Box2D25P and Box2D9P 8192^2 (SplitMix64 uniform grids and weights), Box3D27P 512^3 (the
-seeded sinusoid field), Heat-1D 2^20 (checked against a full oracle sweep, which is cheap in 1D) and
+seeded sinusoid field; at T = 32 the mixed forms against the FP64 form at every point), Heat-1D 2^20 (checked against a full oracle sweep, which is cheap in 1D) and
 the 96-grid batch (FP16 against FP64 on every grid, FP64 bitwise against single-grid plans, the
 smallest grids against the oracle).
 """
@@ -20,7 +20,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-BARS = {"fp64": 1e-12, "f16": 1e-6, "bf16": 1e-6}
+BARS = {"fp64": 1e-12, "f16": 1e-6, "bf16": 1e-6, "f32": 1e-6}
 
 
 def t_fold_kernel(oracle, dims, radius, w, steps):
@@ -135,7 +135,7 @@ def test_8192_fourier_mode(ffs, torch_cuda, form):
     assert err <= BARS[form], (form, info, err)
 
 
-@pytest.mark.parametrize("form", ["bf16", "fp64"])
+@pytest.mark.parametrize("form", ["bf16", "f32", "fp64"])
 def test_512cube_sampled_operator(ffs, oracle, torch_cuda, form):
     # the 3D workload's sinusoid field at 512^3, T = 4 (two blocks of depth 2)
     _, r, w, _ = ffs.named_stencil("Box3D27P", seed=0x5EED0003)
@@ -147,6 +147,27 @@ def test_512cube_sampled_operator(ffs, oracle, torch_cuda, form):
     got = d_out.cpu().numpy()[tuple(pts)]
     err = np.abs(got - ref).max() / np.abs(ref).max()
     assert err <= BARS[form], (form, info, err)
+
+
+def test_512cube_workload_t32(ffs, oracle, torch_cuda):
+    # the 3D workload itself, T = 32 (16 blocks of depth 2): the FP64 form at sampled points against the
+    # 32-fold operator, then each mixed form against the FP64 form at every point
+    _, r, w, _ = ffs.named_stencil("Box3D27P", seed=0x5EED0003)
+    sizes, steps = (512, 512, 512), 32
+    x = sinusoid_field(ffs, 0x5EED0004, sizes)
+    d64, info = run_plan(ffs, torch_cuda, x, r, w, steps, "fp64")
+    pts = sample_points(sizes, 256, 11)
+    ref = apply_at(x, t_fold_kernel(oracle, 3, r, w, steps), pts)
+    got = d64.cpu().numpy()[tuple(pts)]
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err <= BARS["fp64"], ("fp64", info, err)
+    scale = d64.abs().max().item()
+    for form in ("bf16", "f32"):
+        d, info = run_plan(ffs, torch_cuda, x, r, w, steps, form)
+        err = (d - d64).abs().max().item() / scale
+        print(form, err)
+        assert err <= BARS[form], (form, info, err)
+        del d
 
 
 @pytest.mark.parametrize("form,algorithm", [("fp64", "fft"), ("fp64", "direct"), ("bf16", "fft")])

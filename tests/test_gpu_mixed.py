@@ -131,7 +131,7 @@ def test_2d_bf16_tiles(ffs, oracle, torch_cuda, tile, size):
         assert rel <= TOL_MIXED, (form, tile, size, steps, info, rel)
 
 
-@pytest.mark.parametrize("form", ["bf16"])
+@pytest.mark.parametrize("form", ["bf16", "f32"])
 @pytest.mark.parametrize("shape", ["Heat-3D", "Box3D27P"])
 @pytest.mark.parametrize("size", [(16, 16, 16), (11, 13, 17), (40, 24, 30)])
 def test_3d_mixed_named(ffs, oracle, torch_cuda, form, shape, size):
@@ -144,14 +144,21 @@ def test_3d_mixed_named(ffs, oracle, torch_cuda, form, shape, size):
         assert rel <= TOL_MIXED, (form, shape, size, steps, info, rel)
 
 
-def test_3d_f32_form_withdrawn(ffs, torch_cuda):
-    # FP32-operand stages miss 1e-6 on smooth 3D fields (six roundings of the accumulator every 4
-    # products per block): plan creation refuses the form in 3D, and the 3D driver reports it
-    with pytest.raises(ffs.FFSError, match="UNSUPPORTED"):
-        ffs.Plan(3, (16, 16, 16), np.full(27, 1 / 27), 1, 4, precision="f32")
+def test_3d_f32_form_is_compensated(ffs, torch_cuda):
+    # the 3D FP32 form runs two products per real GEMM (FP32 matrix entries and their low-order terms):
+    # twice the matrix flops of the FP32 form in 2D, per 16-point stage
+    w3 = np.full(27, 1 / 27)
+    with ffs.Plan(3, (32, 32, 32), w3, 1, 2, precision="f32", fuse=2) as p32, \
+            ffs.Plan(3, (32, 32, 32), w3, 1, 2, precision="fp64", fuse=2) as p64:
+        assert p32.info["tile"][0] == 16
+        # per pair: 6 stages x 16 slices x 4 real GEMMs x 2 products x 2*16^3 flops
+        pairs = p32.info["flops_per_block"] / (6 * 16 * 4 * 2 * 2 * 16 ** 3)
+        assert pairs == int(pairs) and pairs > 0
+        # FP64: Gauss form, 3 real GEMMs per complex product, same pairs
+        assert p64.info["flops_per_block"] == pairs * 6 * 16 * 3 * 2 * 16 ** 3
 
 
-@pytest.mark.parametrize("form", ["bf16"])
+@pytest.mark.parametrize("form", ["bf16", "f32"])
 def test_3d_mixed_sinusoid_field(ffs, oracle, torch_cuda, form):
     # The 3D workload's field scaled down with the same spectral content: random-phase sinusoids with
     # integer wavevectors up to n/32 per axis (16 on 512^3, 2 on 64^3) plus 1e-3 noise, T = 32.

@@ -247,7 +247,7 @@ def test_graph_cache_cycles_buffer_pairs(ffs, oracle, torch_cuda, precision):
 
 
 @pytest.mark.parametrize("radius", [2, 3])
-@pytest.mark.parametrize("precision,algorithm", [("fp64", "fft"), ("bf16", "fft"), ("fp64", "auto")])
+@pytest.mark.parametrize("precision,algorithm", [("fp64", "fft"), ("bf16", "fft"), ("f32", "fft"), ("fp64", "auto")])
 @pytest.mark.parametrize("convention", ["ref", "centred"])
 def test_3d_wider_stencils(ffs, oracle, torch_cuda, radius, precision, algorithm, convention):
     """Custom 3D stencils wider than the named ones (5^3 and 7^3 random weights): the 16^3 tiles then

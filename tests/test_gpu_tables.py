@@ -3,7 +3,8 @@ forms, so that a wrong DFT entry, sign, index map or normalisation is found in t
 only through an end-to-end result:
   * DFT operands: F[j][k] = exp(-2 pi i j k / L) and F F^H = L I, in every family's per-lane layout
     (reference: src/2D/create_fft_pfa_plan.cu:25-39, src/1D/create_fft_pfa_plan.cu:43-76);
-  * split operands of the mixed forms: the terms sum back to F within the split's precision;
+  * split operands of the mixed forms: the terms sum back to F within the split's precision, and the
+    FP32 operands with their low-order terms give F to ~2^-48;
   * spectra: H = K^t / prod(P) against a numpy FFT of the zero-padded kernel, K(0)^t = (sum w)^t, and
     the Heat-1D closed form K^t = ((1 + cos th)/2)^t e^{-i th t} on the 240-point prime-factor map
     (reference: src/2D/create_fft_pfa_plan.cu:76-107, src/1D/create_fft_pfa_plan.cu:143-178).
@@ -126,6 +127,40 @@ def test_split_operands_32(ffs, torch_cuda, form, bound):
         if form == "f16":
             assert np.all(terms[arr, 2] == 0)  # no third FP16 term
     plan.close()
+
+
+def test_split_operands_16(ffs, torch_cuda):
+    """16-point mixed operands (a 3D plan: every axis 16 points).  BF16 words [axis][arr][pairing][word]
+    [lane], element e of lane (j + 16q) = row 4q + (e & 3), column j; pairings 0, 1, 2 carry the terms
+    h, m, l in their first four elements, which sum back to F within the 3-term split's precision.  FP32
+    chunks [axis][arr][c][lane] = F[4q + c][j] rounded to FP32, then their low-order terms, with which
+    they give F to ~2^-48 (the 3D FP32 form's compensated matrix)."""
+    with ffs.Plan(3, (32, 32, 32), np.full(27, 1 / 27), 1, 2, precision="f32") as plan:
+        words = table(plan, 1, np.uint32)
+    F = dft(16)
+    wants = (F.real, F.imag, -F.imag)
+    bf = words[:3 * 3072].reshape(3, 3, 4, 4, 64)
+    f32 = words[3 * 3072:3 * 3072 + 2 * 3 * 768].view(np.float32).astype(np.float64)
+    hi, lo = f32[:2304].reshape(3, 3, 4, 64), f32[2304:].reshape(3, 3, 4, 64)
+    lanes = np.arange(64)
+    rows_c = 4 * (lanes // 16)
+    cols = lanes % 16
+    for axis in range(3):
+        for arr, want in enumerate(wants):
+            split = np.zeros((16, 16))
+            for g in range(3):
+                for wd in range(2):  # elements 0..3: words 0 and 1
+                    for h in range(2):
+                        bits = ((bf[axis, arr, g, wd] >> (16 * h)) & 0xFFFF).astype(np.uint32) << 16
+                        split[rows_c + 2 * wd + h, cols] += bits.view(np.float32).astype(np.float64)
+            assert np.abs(split - want).max() <= 2.0 ** -24, (axis, arr)
+            H, L = np.zeros((16, 16)), np.zeros((16, 16))
+            for c in range(4):
+                H[rows_c + c, cols] = hi[axis, arr, c]
+                L[rows_c + c, cols] = lo[axis, arr, c]
+            assert np.abs(H - want).max() <= 2.0 ** -25, (axis, arr)  # FP32 rounding of entries <= 1
+            assert np.abs(H + L - want).max() <= 2.0 ** -46, (axis, arr)
+            assert np.abs(H - want).max() > 2.0 ** -30  # the low-order terms carry something
 
 
 def spectrum(plan, length):
