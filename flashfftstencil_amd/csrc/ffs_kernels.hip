@@ -1729,10 +1729,14 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
       }
     }
     __syncthreads();
-    // ---- phase 3: slices n0 = 4w + s; inverse axis 2; store ----
+    // ---- phase 3: the slices n0 of the valid block only, h0 + w + 4s (V0 of the 16, fewer at the grid's
+    // far edge, spread over the four waves: 3 each at V0 = 12); inverse axis 2; store ----
+    const int nvalid0 = (int)(a.V0 < a.n0 - o0 ? a.V0 : a.n0 - o0);
 #pragma unroll kSlices
     for (int s = 0; s < 4; s++) {
-      const int n0 = 4 * w + s;
+      const int i0 = w + 4 * s;
+      if (i0 >= nvalid0) break;
+      const int n0 = (int)a.h0 + i0;
       V4 ar, ai;
 #pragma unroll
       for (int x = 0; x < 4; x++) {
@@ -1742,8 +1746,8 @@ __global__ __launch_bounds__(256) void k_tile3d(TileArgs a) {
       }
       V4 br, bi;
       P::template stage<true>(ar, ai, f, br, bi);  // [n1 = row][n2 = lane]
-      const int64_t y0 = o0 + n0 - a.h0;
-      if (n0 >= a.h0 && n0 < a.h0 + a.V0 && y0 < a.n0) {
+      const int64_t y0 = o0 + i0;
+      {
         const bool colok = j >= a.h2 && j < a.h2 + a.V2;
         const int64_t zA = o2A + j - a.h2, zB = o2B + j - a.h2;
         const bool okA = colok && zA < a.n2, okB = colok && hasB && zB < a.n2;
