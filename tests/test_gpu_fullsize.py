@@ -165,8 +165,12 @@ def test_512cube_workload_t32(ffs, oracle, torch_cuda):
     for form in ("bf16", "f32"):
         d, info = run_plan(ffs, torch_cuda, x, r, w, steps, form)
         err = (d - d64).abs().max().item() / scale
-        print(form, err)
+        bias = (d - d64).mean().item() / scale
+        print(form, err, bias)
         assert err <= BARS[form], (form, info, err)
+        # no drift of the mean: the BF16 MFMA truncates a small accumulator toward zero, which, chained
+        # through the split products of every stage, once biased the result by -1.3e-7 here
+        assert abs(bias) <= 5e-8, (form, info, bias)
         del d
 
 
