@@ -910,6 +910,8 @@ ffs_status ffs_plan_get_info(const ffs_plan* pl, ffs_plan_info* info) {
   if (p.precision == FFS_PREC_MIXED_BF16) per_cplx = 4.0 * 3.0 * 2.0 * 16 * 16 * 32;
   if (p.precision == FFS_PREC_MIXED_F32) per_cplx = (p.dims == 3 ? 8.0 : 4.0) * gemm;
   double per_pair = 2.0 * dd * per_cplx * std::pow(16.0, dd - 2);
+  // 3D: the last inverse stage (phase 3) runs only the V0 slices of the valid block, not all 16
+  if (pl->kind == ffsk::KIND_3D16) per_pair = (5.0 * 16.0 + std::min(16, pl->V[0])) * per_cplx;
   // 32-point family: output blocks over the 4 stages (4 + 4 + 4 + 4, or 4 + 4 + 2 + 1 for an out16 main
   // block) x 4 real GEMMs x (6, 4 or 3) K=32 MFMAs
   if (pl->kind == ffsk::KIND_2D32 && p.precision == FFS_PREC_FP64)  // x 3 Gauss GEMMs x 8 K=4 chunks

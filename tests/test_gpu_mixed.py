@@ -151,11 +151,14 @@ def test_3d_f32_form_is_compensated(ffs, torch_cuda):
     with ffs.Plan(3, (32, 32, 32), w3, 1, 2, precision="f32", fuse=2) as p32, \
             ffs.Plan(3, (32, 32, 32), w3, 1, 2, precision="fp64", fuse=2) as p64:
         assert p32.info["tile"][0] == 16
-        # per pair: 6 stages x 16 slices x 4 real GEMMs x 2 products x 2*16^3 flops
-        pairs = p32.info["flops_per_block"] / (6 * 16 * 4 * 2 * 2 * 16 ** 3)
+        # per pair: (5 x 16 + V = 12) slice stages (the last inverse stage runs only the valid slices)
+        # x 4 real GEMMs x 2 products x 2*16^3 flops
+        assert p32.info["valid"][0] == 12
+        stages = 5 * 16 + 12
+        pairs = p32.info["flops_per_block"] / (stages * 4 * 2 * 2 * 16 ** 3)
         assert pairs == int(pairs) and pairs > 0
         # FP64: Gauss form, 3 real GEMMs per complex product, same pairs
-        assert p64.info["flops_per_block"] == pairs * 6 * 16 * 3 * 2 * 16 ** 3
+        assert p64.info["flops_per_block"] == pairs * stages * 3 * 2 * 16 ** 3
 
 
 @pytest.mark.parametrize("form", ["bf16", "f32"])
