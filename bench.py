@@ -181,11 +181,20 @@ def oracle_sample(x, radius, w, T, seconds, label):
     from tests import _oracle
 
     threads = cpu_threads()
-    t1 = time.perf_counter()
-    _oracle.sweep_fast(x, radius, w, 1, threads=threads)
-    one = time.perf_counter() - t1
-    ts = int(max(1, min(T, math.floor(seconds / max(one, 1e-6)))))
-    reps = int(max(1, min(1000, math.floor(seconds / max(one * ts, 1e-6))))) if ts == T else 1
+
+    def timed(k):
+        t1 = time.perf_counter()
+        _oracle.sweep_fast(x, radius, w, k, threads=threads)
+        return time.perf_counter() - t1
+
+    timed(1)  # first call: thread pool start-up
+    # cost model of one call, per_call + k * per_step (the call allocates its result and starts threads)
+    k = min(T, 8)
+    c1, ck = timed(1), timed(k)
+    per_step = max((ck - c1) / (k - 1), 1e-7) if k > 1 else c1
+    per_call = max(c1 - per_step, 0.0)
+    ts = int(max(1, min(T, math.floor(max(seconds - per_call, per_step) / per_step))))
+    reps = int(max(1, min(100000, math.floor(seconds / (per_call + per_step * ts))))) if ts == T else 1
     t1 = time.perf_counter()
     for _ in range(reps):
         ref = _oracle.sweep_fast(x, radius, w, ts, threads=threads)
@@ -480,7 +489,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="all", choices=("all",) + ORDER)
     ap.add_argument("--kernel-reps", type=int, default=10, help="timed single-block launches for the roofline")
-    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="target length of each CPU-baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of each CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines and the oracle sample checks")
     ap.add_argument("--forms", default=None, help="comma list overriding the workloads' forms")
     ap.add_argument("--no-fft-baseline", action="store_true", help="skip the rocFFT and library-convolution baselines")
