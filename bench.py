@@ -188,16 +188,18 @@ def oracle_sample(x, radius, w, T, seconds, label):
         return time.perf_counter() - t1
 
     timed(1)  # first call: thread pool start-up
-    # cost model of one call, per_call + k * per_step (the call allocates its result and starts threads)
+    # steps that fit the target (per step from an 8-step call, which also pays one allocation: an upper
+    # bound); when all T steps fit, whole T-step runs are repeated until the target time has passed
     k = min(T, 8)
-    c1, ck = timed(1), timed(k)
-    per_step = max((ck - c1) / (k - 1), 1e-7) if k > 1 else c1
-    per_call = max(c1 - per_step, 0.0)
-    ts = int(max(1, min(T, math.floor(max(seconds - per_call, per_step) / per_step))))
-    reps = int(max(1, min(100000, math.floor(seconds / (per_call + per_step * ts))))) if ts == T else 1
+    per_step = timed(k) / k
+    ts = int(max(1, min(T, math.floor(seconds / per_step))))
+    reps = 0
     t1 = time.perf_counter()
-    for _ in range(reps):
+    while True:
         ref = _oracle.sweep_fast(x, radius, w, ts, threads=threads)
+        reps += 1
+        if ts < T or time.perf_counter() - t1 >= seconds or reps >= 100000:
+            break
     cpu_s = time.perf_counter() - t1
     cpu = {"value": x.size * ts * reps / cpu_s / 1e9, "unit": "GStencils/s", "cores": threads, "kind": "port",
            "sample": f"{label}, {ts} of the {T} steps" + (f" repeated {reps} times" if reps > 1 else "")
