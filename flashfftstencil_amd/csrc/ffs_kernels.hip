@@ -730,34 +730,43 @@ __device__ __forceinline__ void mstage(const f4& ar, const f4& ai, const OpsBF16
   outi = (ri + ii) + im;
 }
 
-template <bool INV>
+template <bool INV, int C0 = 0, int C1 = 4>
 __device__ __forceinline__ void mstage_f32(const f4& ar, const f4& ai, const OpsF32& f, f4& re, f4& im) {
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
+  for (int c = C0; c < C1; c++) {
     re = FFS_MFMA_F32(ar[c], f.r[c], re);
     im = FFS_MFMA_F32(ar[c], INV ? f.ni[c] : f.i[c], im);
     re = FFS_MFMA_F32(ai[c], INV ? f.i[c] : f.ni[c], re);
     im = FFS_MFMA_F32(ai[c], f.r[c], im);
   }
 }
+// FP32 stage: v_mfma_f32_16x16x4f32 rounds every product and every addition (in sequence) to FP32, so
+// chunks 0-1 and 2-3 (8 of the 16 K rows each) go to two accumulators, added at the end: 16 roundings
+// in each chain instead of 32 in one.
 template <bool INV, int NTERMS>
 __device__ __forceinline__ void mstage(const f4& ar, const f4& ai, const OpsF32& f, f4& outr, f4& outi) {
-  f4 re = {0.f, 0.f, 0.f, 0.f}, im = re;
-  mstage_f32<INV>(ar, ai, f, re, im);
-  outr = re;
-  outi = im;
+  const f4 z = {0.f, 0.f, 0.f, 0.f};
+  f4 re = z, im = z, re2 = z, im2 = z;
+  mstage_f32<INV, 0, 2>(ar, ai, f, re, im);
+  mstage_f32<INV, 2, 4>(ar, ai, f, re2, im2);
+  outr = re + re2;
+  outi = im + im2;
 }
 // Compensated FP32 stage: the products with the matrix's low-order terms first (about 2^-24 of the
-// result, so their roundings are negligible), then those with its FP32-rounded entries, into the same
-// FP32 accumulators.  The matrix then enters to ~2^-48 instead of FP32's 2^-25: its rounding is the same
-// in every window and every block, so uncompensated it adds up coherently over a run.
+// result, so their roundings are negligible), then those with its FP32-rounded entries.  The matrix then
+// enters to ~2^-48 instead of FP32's 2^-25: its rounding is the same in every window and every block, so
+// uncompensated it adds up coherently over a run.  What remains is the rounding along the accumulation
+// chain, halved as in the FP32 stage above: the main products of chunks 0-1 and 2-3 go to two
+// accumulators, added at the end.
 template <bool INV, int NTERMS>
 __device__ __forceinline__ void mstage(const f4& ar, const f4& ai, const OpsF32C& f, f4& outr, f4& outi) {
-  f4 re = {0.f, 0.f, 0.f, 0.f}, im = re;
+  const f4 z = {0.f, 0.f, 0.f, 0.f};
+  f4 re = z, im = z, re2 = z, im2 = z;
   mstage_f32<INV>(ar, ai, f.lo, re, im);
-  mstage_f32<INV>(ar, ai, f.hi, re, im);
-  outr = re;
-  outi = im;
+  mstage_f32<INV, 0, 2>(ar, ai, f.hi, re, im);
+  mstage_f32<INV, 2, 4>(ar, ai, f.hi, re2, im2);
+  outr = re + re2;
+  outi = im + im2;
 }
 
 // Load the pair's windows in the FP32-accumulator order: lane (j + 16q), element r = window[4q + r][j].

@@ -21,7 +21,7 @@ def _case(rng):
     elif dims == 2:
         sizes, precs = (rng.randint(3, 150), rng.randint(3, 150)), ["fp64", "f16", "bf16", "f32"]
     else:
-        sizes, precs = (rng.randint(3, 40), rng.randint(3, 40), rng.randint(3, 40)), ["fp64", "bf16"]
+        sizes, precs = (rng.randint(3, 40), rng.randint(3, 40), rng.randint(3, 40)), ["fp64", "bf16", "f32"]
     kw = dict(precision=rng.choice(precs), convention=rng.choice(["ref", "centred"]))
     if dims == 2 and kw["precision"] == "fp64" and rng.random() < 0.5:
         kw["tile"] = rng.choice([16, 32])
